@@ -1,0 +1,201 @@
+/*
+ * nnsph.h -- C-ABI of libnnsph.so, the MI355X-native SPH step of chunleili/nonNewton.
+ *
+ * The reference has no FFI: its boundary is the Python object surface of
+ * SPH/main.py (README.md:25-33).  Each entry point below replaces one piece of
+ * that surface; the citation names it.  The Python mirror nonnewton_amd/sph.py
+ * binds these with ctypes and keeps the reference's names.
+ *
+ * Conventions
+ *  - Every function returns an int32 status: 0 = OK, negative = error (NN_E_*),
+ *    NN_W_NOCONV (positive) = a solver hit its iteration limit.
+ *    nn_last_error() gives the text of the last error on the calling thread.
+ *  - The library owns all device memory; it allocates only in nn_create,
+ *    nn_upload and nn_pool_* admission/tick.  Host pointers are borrowed for the
+ *    duration of the call.
+ *  - Internally particles are kept in cell order; every array crossing this
+ *    boundary is in the caller's original particle order.
+ *  - A context is not thread-safe; distinct contexts are independent.
+ *  - Arithmetic is FP32 per particle; solver-termination reductions and CG
+ *    scalars are FP64.
+ */
+#ifndef NNSPH_H
+#define NNSPH_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NNSPH_ABI_VERSION 1
+
+/* status codes */
+#define NN_OK 0
+#define NN_E_ARG (-1)
+#define NN_E_HIP (-2)
+#define NN_E_OOM (-3)
+#define NN_E_NEIGHBOR_OVERFLOW (-4) /* > 60 neighbors (max_num_neighbors, SPH/main.py:777) */
+#define NN_E_CELL_OVERFLOW (-5)     /* > 50 particles in a cell (:778) */
+#define NN_E_OUT_OF_GRID (-6)       /* particle outside the grid (pos_to_index, :805-807) */
+#define NN_E_NAN (-7)               /* non-finite position or velocity after a step */
+#define NN_E_UNSUPPORTED (-8)
+#define NN_E_BUSY (-9)              /* pool admission: over budget */
+#define NN_E_STATE (-10)            /* call out of order (e.g. step before init) */
+#define NN_E_REST_OVERFLOW (-11)    /* > 50 rest neighbors of an elastic particle (:1192) */
+#define NN_W_NOCONV 1               /* a solver stopped at its iteration limit */
+
+/* particle kinds (material + solid type, SPH/main.py:27-40, :258-272) */
+#define NN_KIND_FLUID 0
+#define NN_KIND_STATIC 1
+#define NN_KIND_ELASTIC 2
+
+/* Scene parameters: Parameter (SPH/main.py:598-621), DFSPHSolver.__init__
+ * (:1520-1544), NonNewton.initParameters (:358-371), SurfaceTension (:1382),
+ * Elasticity.__init__ (:1188-1189), Viscosity_Weiler2018.__init__ (:1480-1481). */
+typedef struct nn_params {
+    double domain_start[3];
+    double domain_end[3];
+    double particle_radius;
+    double density0;
+    double gravity[3];
+    double boundary_viscosity;
+    double dt;
+    double sticky_coeff;
+    double collision_coeff;
+    double viscosity;
+    int32_t max_iterations_v;
+    int32_t max_iterations;
+    double max_error_v;
+    double max_error;
+    double eps;
+    int32_t enable_divergence_solver;
+    int32_t use_surface_tension;
+    int32_t use_viscosity;
+    int32_t use_non_newtonian;
+    int32_t viscosity_method; /* 0 explicit (ViscosityModel), 1 Weiler 2018 implicit */
+    int32_t non_newton_method; /* 0..6, NonNewton.NON_NEWTON_METHOD (:377-384) */
+    double surface_tension;
+    double power_index;
+    double consistency_index;
+    double viscosity0;
+    double viscosity_inf;
+    double critical_strain_rate;
+    double mu_c;
+    double yield_stress;
+    double max_viscosity;
+    int32_t weiler_max_iter;
+    int32_t compensated; /* Kahan sums for density and boundary volume */
+    double weiler_tol;
+    double youngs_modulus;
+    double poisson_ratio;
+    double particle_mass; /* <= 0: volume0 * 1000 (:743) */
+    int32_t warm_start;   /* reserved, must be 0 */
+    int32_t reserved;
+} nn_params;
+
+/* Per-step statistics (what the reference prints each step, :916,1548,1704,1785). */
+typedef struct nn_step_stats {
+    int32_t iterations_v;  /* divergence-solver iterations run */
+    int32_t iterations_p;  /* pressure-solver iterations run */
+    int32_t iterations_cg; /* Weiler CG iterations */
+    int32_t status;        /* NN status bits of the step (1 nbr, 2 cell, 4 grid, 8 nan, 16 rest) */
+    double avg_density_err_v;
+    double avg_density_err_p;
+    double cg_rel_residual;
+    double max_density;
+    int64_t num_pairs;     /* sum of neighbor counts */
+} nn_step_stats;
+
+/* field ids for nn_download / nn_upload_field (ParticleData, :701-723) */
+#define NN_FIELD_X 0            /* float[n][3] */
+#define NN_FIELD_V 1            /* float[n][3] */
+#define NN_FIELD_ACCELERATION 2 /* float[n][3] */
+#define NN_FIELD_VOLUME 3       /* float[n] */
+#define NN_FIELD_DENSITY 4      /* float[n] */
+#define NN_FIELD_DFSPH_FACTOR 5 /* float[n], as the reference holds it after a step */
+#define NN_FIELD_DENSITY_ADV 6  /* float[n] */
+#define NN_FIELD_STRAIN_NORM 7  /* float[n] (NonNewton.strainRateNorm, :340) */
+#define NN_FIELD_VISCOSITY 8    /* float[n] (NonNewton.nonNewtonViscosity, :342) */
+#define NN_FIELD_ROTATION 9     /* float[n][9] (Elasticity.m_rotations, :1199) */
+#define NN_FIELD_STRESS 10      /* float[n][6] (Elasticity.m_stress, :1200) */
+#define NN_FIELD_REST_VOLUME 12 /* float[n] (Elasticity.m_restVolumes, :1198) */
+#define NN_FIELD_X0 13          /* float[n][3] (ParticleData.x_0) */
+#define NN_FIELD_KIND 14        /* int32[n] */
+#define NN_FIELD_NUM_NEIGHBORS 15 /* int32[n] (NeighborhoodSearchSparse.num_neighbors) */
+
+typedef struct nn_ctx nn_ctx;
+typedef struct nn_pool nn_pool;
+
+/* Library identity: ABI version and the device it runs on. */
+int32_t nn_version(void);
+const char *nn_last_error(void);
+int32_t nn_device_info(char *name, int32_t name_len, int64_t *hbm_bytes, int32_t *num_cus);
+
+/* Context for one scene -- replaces initialize() + ParticleData(N) +
+ * NeighborhoodSearchSparse(...) (SPH/main.py:1870-1880, :701-723, :766-791). */
+int32_t nn_create(const nn_params *params, int64_t max_particles, nn_ctx **out);
+int32_t nn_destroy(nn_ctx *ctx);
+
+/* Particle upload -- replaces initialize_particles (SPH/main.py:735-758):
+ * x, v float[n][3], kind int32[n] (NN_KIND_*); volume = volume0, m = mass. */
+int32_t nn_upload(nn_ctx *ctx, const float *x, const float *v, const int32_t *kind, int64_t n);
+
+/* Replaces Elasticity(N).initValues (:1216-1236) + SPHBase.initialize (:957-960). */
+int32_t nn_init(nn_ctx *ctx);
+
+/* Replaces SPHBase.step (:915-940), run nsteps times; stats of the last step. */
+int32_t nn_step(nn_ctx *ctx, int32_t nsteps, nn_step_stats *out);
+
+/* Replaces meta.ns.run_search() (:822-828) on the current positions. */
+int32_t nn_search(nn_ctx *ctx);
+
+/* Replaces meta.pd.<field>.to_numpy() / .from_numpy() (:740-741, :940, :1548).
+ * bytes must equal n * width * 4. */
+int32_t nn_download(nn_ctx *ctx, int32_t field, void *host, size_t bytes);
+int32_t nn_upload_field(nn_ctx *ctx, int32_t field, const void *host, size_t bytes);
+
+/* Replaces meta.ns.neighbors / num_neighbors (:780-781): nbr int32[n][cap],
+ * -1 padded, original indices, in the reference's serial list order. */
+int32_t nn_neighbors(nn_ctx *ctx, int32_t *nbr, int32_t *count, int32_t cap);
+
+/* Replaces the 0-d fields meta.parm.dt[None] / viscosity[None] (:611-612,
+ * :620-621) that the reference mutates at run time (:1905).  key: "dt",
+ * "viscosity", "boundary_viscosity", "max_error", "max_error_v". */
+int32_t nn_set_param(nn_ctx *ctx, const char *key, double value);
+
+/* Number of particles and of fluid particles (meta.particle_max_num,
+ * meta.fluid_particle_num, :1873). */
+int32_t nn_counts(nn_ctx *ctx, int64_t *n, int64_t *n_fluid);
+
+/* ---------------------------------------------------------------------------
+ * Batched / serving form (no counterpart in the reference, whose scenes are
+ * single; this is the multi-scene driver over the same step).  A pool holds
+ * many independent scenes resident in HBM in one concatenated, offset-indexed
+ * particle array and advances all of them with one launch sequence per tick.
+ * ------------------------------------------------------------------------- */
+int32_t nn_pool_create(int64_t budget_bytes, nn_pool **out);
+int32_t nn_pool_destroy(nn_pool *pool);
+/* Bytes of device memory a scene of n particles (n_elastic elastic) occupies. */
+int64_t nn_pool_footprint(int64_t n, int64_t n_elastic, const nn_params *params);
+/* Admit a scene: uploaded and initialized now (on a side stream, overlapping a
+ * running tick), joins the batch at the next tick.  NN_E_BUSY if over budget. */
+int32_t nn_pool_admit(nn_pool *pool, const nn_params *params, const float *x, const float *v,
+                      const int32_t *kind, int64_t n, int32_t *slot);
+/* One step of every resident scene; per-slot stats into stats[slot] when given
+ * (array of nn_pool_capacity() entries).  Returns the number of scenes stepped
+ * through *stepped. */
+int32_t nn_pool_tick(nn_pool *pool, nn_step_stats *stats, int32_t stats_len, int32_t *stepped);
+int32_t nn_pool_retire(nn_pool *pool, int32_t slot);
+int32_t nn_pool_read(nn_pool *pool, int32_t slot, int32_t field, void *host, size_t bytes);
+int32_t nn_pool_info(nn_pool *pool, int32_t *resident, int32_t *pending, int64_t *bytes_in_use);
+int32_t nn_pool_capacity(void);
+/* Launch shape used by the sweep kernels (block size, power of two 64..1024);
+ * the autotuner sets it.  0 = library default. */
+int32_t nn_pool_set_block(nn_pool *pool, int32_t block);
+int32_t nn_set_block(nn_ctx *ctx, int32_t block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NNSPH_H */

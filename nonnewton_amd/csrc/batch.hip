@@ -1,0 +1,889 @@
+// batch.hip -- host driver of the SPH step: derived constants, device layout of
+// one or many scenes, upload, initialization, the per-step launch sequence with
+// device-side solver termination, and downloads in original particle order.
+#include "batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace nn {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+const char* last_error_text() { return g_err.c_str(); }
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return NN_E_HIP;
+}
+
+#define HIPCHK(expr)                                   \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+#define CHK(expr)              \
+  do {                         \
+    int _rc = (expr);          \
+    if (_rc < 0) return _rc;   \
+  } while (0)
+
+static inline int round_up64(int v) { return (v + 63) & ~63; }
+
+// cubic_kernel (SPH/main.py:859-880), host copy of the device function
+static float host_kern_w(float h, float kW, float rn) {
+  float res = 0.0f;
+  float q = rn / h;
+  if (q <= 1.0f) {
+    if (q <= 0.5f) {
+      float q2 = q * q;
+      float q3 = q2 * q;
+      res = kW * (6.0f * q3 - 6.0f * q2 + 1.0f);
+    } else {
+      float t = 1.0f - q;
+      res = (kW * 2.0f) * (t * t * t);
+    }
+  }
+  return res;
+}
+
+float neighbor_threshold(float h) {
+  float t = h * h;
+  while (t > 0.0f && sqrtf(t) >= h) t = nextafterf(t, 0.0f);
+  while (sqrtf(t) < h) t = nextafterf(t, INFINITY);
+  return t;
+}
+
+int derive_scene(SceneHost& s, std::string& err) {
+  const nn_params& p = s.params;
+  if (!(p.particle_radius > 0.0) || !(p.dt > 0.0) || !(p.density0 > 0.0)) {
+    err = "particle_radius, dt and density0 must be positive";
+    return NN_E_ARG;
+  }
+  if (p.warm_start) {
+    err = "warm_start is not supported";
+    return NN_E_UNSUPPORTED;
+  }
+  if (p.non_newton_method < 0 || p.non_newton_method > 6) {
+    err = "non_newton_method must be 0..6";
+    return NN_E_ARG;
+  }
+  const double r = p.particle_radius, dd = 2.0 * r, h = 4.0 * r;
+  long cells = 1;
+  for (int k = 0; k < 3; k++) {
+    const double g = std::ceil((p.domain_end[k] - p.domain_start[k]) / h);  // :773-775
+    if (!(g >= 1.0) || g > 1.0e6) {
+      err = "domain must span at least one cell per axis";
+      return NN_E_ARG;
+    }
+    s.gn[k] = (int)g;
+    cells *= s.gn[k];
+  }
+  if (cells > (1L << 30)) {
+    err = "grid too large";
+    return NN_E_ARG;
+  }
+  s.ncells = cells;
+  SceneDev& c = s.dev;
+  c.gn0 = s.gn[0];
+  c.gn1 = s.gn[1];
+  c.gn2 = s.gn[2];
+  c.h = (float)h;
+  c.thr2 = neighbor_threshold(c.h);
+  c.kW = (float)(8.0 / M_PI / (h * h * h));
+  c.kG = (float)(6.0 * 8.0 / M_PI / (h * h * h));
+  c.W0 = host_kern_w(c.h, c.kW, 0.0f);
+  c.dt = (float)p.dt;
+  c.inv_dt = (float)(1.0 / (double)c.dt);
+  c.inv_dt2 = (float)(1.0 / ((double)c.dt * (double)c.dt));
+  c.rho0 = (float)p.density0;
+  const double vol0 = 0.8 * dd * dd * dd;  // :607
+  c.mass = (float)(p.particle_mass > 0.0 ? p.particle_mass : vol0 * 1000.0);  // :743
+  c.nu = (float)p.viscosity;
+  c.bnu = (float)p.boundary_viscosity;
+  c.gx = (float)p.gravity[0];
+  c.gy = (float)p.gravity[1];
+  c.gz = (float)p.gravity[2];
+  c.lo0 = (float)(p.domain_start[0] + h);
+  c.lo1 = (float)(p.domain_start[1] + h);
+  c.lo2 = (float)(p.domain_start[2] + h);
+  c.hi0 = (float)(p.domain_end[0] - h);
+  c.hi1 = (float)(p.domain_end[1] - h);
+  c.hi2 = (float)(p.domain_end[2] - h);
+  c.diam2 = (float)(dd * dd);
+  const float dr = (float)dd;
+  c.Wd = host_kern_w(c.h, c.kW, sqrtf(dr * dr + 0.0f * 0.0f + 0.0f * 0.0f));  // :1414
+  c.visc_eps = (float)(0.01 * h * h);
+  c.sigma = (float)p.surface_tension;
+  c.opc = (float)(1.0 + p.collision_coeff);
+  c.sticky = (float)p.sticky_coeff;
+  c.eps = (float)p.eps;
+  const double E = p.youngs_modulus, nu = p.poisson_ratio;
+  const double factor = E / ((1.0 + nu) * (1.0 - 2.0 * nu));  // :1277-1281
+  c.C_diag = (float)(factor * (1.0 - nu));
+  c.C_off = (float)(factor * nu);
+  c.C_shear = (float)(factor * 0.5 * (1.0 - 2.0 * nu));
+  c.nn_method = p.non_newton_method;
+  c.nn_n = (float)p.power_index;
+  c.nn_K = (float)p.consistency_index;
+  c.nn_e0 = (float)p.viscosity0;
+  c.nn_einf = (float)p.viscosity_inf;
+  c.nn_gc = (float)p.critical_strain_rate;
+  c.nn_muc = (float)p.mu_c;
+  c.nn_tauy = (float)p.yield_stress;
+  c.nn_max = (float)p.max_viscosity;
+  c.ten_nu = 10.0f * c.nu;
+  c.ten_bnu = 10.0f * c.bnu;
+  c.sig_m = c.sigma / c.mass * c.mass;
+  c.eta_v = 1.0 / (double)c.dt * p.max_error_v * 0.01 * p.density0;  // :1699
+  c.eta_p = p.max_error * 0.01 * p.density0;                          // :1781
+  c.cg_tol2 = p.weiler_tol * p.weiler_tol;
+  c.rho0_d = p.density0;
+  c.max_iter_v = p.max_iterations_v;
+  c.max_iter_p = p.max_iterations;
+  c.max_iter_cg = p.weiler_max_iter;
+  int f = 0;
+  if (p.use_surface_tension) f |= SF_SURFACE_TENSION;
+  if (p.use_viscosity) f |= SF_VISCOSITY;
+  if (p.use_non_newtonian) f |= SF_NON_NEWTONIAN;
+  if (p.viscosity_method == 1) f |= SF_WEILER;
+  else if (p.viscosity_method != 0) {
+    err = "viscosity_method must be 0 (explicit) or 1 (Weiler)";
+    return NN_E_ARG;
+  }
+  if (p.enable_divergence_solver) f |= SF_DIVERGENCE;
+  if (p.compensated) f |= SF_COMPENSATED;
+  if (s.n_fluid > 0) f |= SF_HAS_FLUID;
+  if (s.n_elastic > 0) f |= SF_HAS_ELASTIC;
+  c.flags = f;
+  c.n = s.n;
+  c.n_fluid = s.n_fluid;
+  c.n_elastic = s.n_elastic;
+  c.nf_d = (double)s.n_fluid;
+  return NN_OK;
+}
+
+// bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wx,wr,wz,wp,wp_alt,wq,wd0,x0 (13 float4),
+// wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int), rho,alpha,src,mu,gam,eta (6 float),
+// nbr (60 int), partials (3 doubles per 64 slots), wave table
+static constexpr long long BYTES_PER_SLOT = 13 * 16 + 8 + 8 * 4 + 6 * 4 + MAX_NB * 4 + 1;
+static constexpr long long BYTES_PER_MEMBER = 4 + 4 + MAX_REST * 4 + 4 * 16 + 1;
+
+long long scene_footprint(long long n, long long ns, long long cells) {
+  const long long np = (n + 63) / 64 * 64, nsp = (ns + 63) / 64 * 64;
+  return np * BYTES_PER_SLOT + nsp * BYTES_PER_MEMBER + (cells + 1) * 8 + 2048;
+}
+
+Batch::Batch(hipStream_t st) : stream(st) {}
+
+Batch::~Batch() {
+  release();
+  if (h_state_) (void)hipHostFree(h_state_);
+  if (h_active_) (void)hipHostFree(h_active_);
+}
+
+template <class T>
+int Batch::alloc(T*& p, size_t count) {
+  p = nullptr;
+  if (count == 0) count = 1;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return NN_E_OOM;
+  }
+  allocs_.push_back({q, count * sizeof(T)});
+  bytes_ += (long long)(count * sizeof(T));
+  p = (T*)q;
+  return NN_OK;
+}
+
+void Batch::release() {
+  for (auto& a : allocs_) (void)hipFree(a.p);
+  allocs_.clear();
+  bytes_ = 0;
+}
+
+void Batch::swap_sort_buffers() {
+  std::swap(d.pos, d.pos_alt);
+  std::swap(d.vel, d.vel_alt);
+  std::swap(d.info, d.info_alt);
+}
+
+int Batch::layout(const std::vector<const Batch*>& from) {
+  struct Src {
+    const Batch* b;
+    int pb, sb;
+  };
+  std::vector<Src> src(scenes.size(), Src{nullptr, 0, 0});
+  for (size_t k = 0; k < scenes.size(); k++) {
+    for (const Batch* b : from) {
+      if (!b) continue;
+      for (auto& t : b->scenes)
+        if (t.get() == scenes[k].get()) src[k] = Src{b, scenes[k]->p_begin, scenes[k]->s_begin};
+    }
+  }
+  // offsets
+  int P = 0, C = 0, NS = 0;
+  long base = 0;
+  any_fluid_ = any_elastic_ = any_weiler_ = any_strain_ = any_div_ = false;
+  max_iter_[0] = max_iter_[1] = max_iter_[2] = 1;
+  for (auto& sp : scenes) {
+    SceneHost& s = *sp;
+    s.p_begin = P;
+    s.dev.p_begin = P;
+    s.dev.shift = (int)(P - base);
+    s.dev.wave_begin = P / 64;
+    base += s.n;
+    P += round_up64(s.n);
+    s.dev.wave_end = P / 64;
+    s.cell_begin = C;
+    s.dev.cell_begin = C;
+    C += (int)s.ncells + 1;
+    s.s_begin = NS;
+    s.dev.s_begin = NS;
+    s.dev.ns = s.ns;
+    NS += round_up64(s.ns);
+    const int f = s.dev.flags;
+    if (f & SF_HAS_FLUID) {
+      any_fluid_ = true;
+      if (f & SF_DIVERGENCE) any_div_ = true;
+      if ((f & SF_VISCOSITY) && (f & SF_WEILER)) any_weiler_ = true;
+      if ((f & SF_NON_NEWTONIAN) || ((f & SF_VISCOSITY) && (f & SF_WEILER))) any_strain_ = true;
+    }
+    if (f & SF_HAS_ELASTIC) any_elastic_ = true;
+    max_iter_[0] = std::max(max_iter_[0], s.dev.max_iter_v);
+    max_iter_[1] = std::max(max_iter_[1], s.dev.max_iter_p);
+    max_iter_[2] = std::max(max_iter_[2], s.dev.max_iter_cg);
+  }
+  // new arrays
+  Dev nd{};
+  std::vector<Alloc> old_allocs;
+  old_allocs.swap(allocs_);
+  long long old_bytes = bytes_;
+  bytes_ = 0;
+  nd.P = P;
+  nd.W = P / 64;
+  nd.C = C;
+  nd.NS = NS;
+  nd.SW = NS / 64;
+  nd.nscenes = (int)scenes.size();
+  C_total_ = C;
+  const int ncell_entries = C + 1;
+  scan_blocks_ = (ncell_entries + 2047) / 2048;
+  int rc = NN_OK;
+  auto A = [&](auto*& p, size_t count) {
+    if (rc == NN_OK) rc = alloc(p, count);
+  };
+  SceneDev* scenes_dev;
+  A(scenes_dev, scenes.size());
+  nd.scenes = scenes_dev;
+  A(nd.state, scenes.size());
+  int* ws;
+  A(ws, nd.W);
+  nd.wave_scene = ws;
+  int* sws;
+  A(sws, nd.SW);
+  nd.s_wave_scene = sws;
+  A(nd.pos, P);
+  A(nd.pos_alt, P);
+  A(nd.vel, P);
+  A(nd.vel_alt, P);
+  A(nd.info, P);
+  A(nd.info_alt, P);
+  A(nd.acc, P);
+  A(nd.rho, P);
+  A(nd.alpha, P);
+  A(nd.src, P);
+  A(nd.nbr, (size_t)P * MAX_NB);
+  A(nd.nnbr, P);
+  A(nd.key, P);
+  A(nd.rank, P);
+  A(nd.inv, P);
+  A(nd.bucket, P);
+  A(nd.cell_count, ncell_entries);
+  A(nd.cell_start, ncell_entries);
+  A(nd.scan_tmp, scan_blocks_);
+  A(nd.partials, 3 * (size_t)nd.W);
+  A(nd.n_active, 4);
+  A(nd.wx, P);
+  A(nd.wr, P);
+  A(nd.wz, P);
+  A(nd.wp, P);
+  A(nd.wp_alt, P);
+  A(nd.wq, P);
+  A(nd.wd0, P);
+  A(nd.wd1, P);
+  A(nd.mu, P);
+  A(nd.gam, P);
+  A(nd.eta, P);
+  A(nd.x0, P);
+  A(nd.s_of, P);
+  A(nd.s_pid, NS);
+  A(nd.rest, (size_t)NS * MAX_REST);
+  A(nd.rest_cnt, NS);
+  A(nd.rs, 4 * (size_t)NS);
+  if (rc != NN_OK) {
+    release();
+    allocs_.swap(old_allocs);
+    bytes_ = old_bytes;
+    return rc;
+  }
+  if (!h_state_) {
+    HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * 4096, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
+  }
+  // tables
+  std::vector<int> hw(std::max(nd.W, 1)), hsw(std::max(nd.SW, 1));
+  std::vector<SceneDev> hs(scenes.size());
+  for (size_t k = 0; k < scenes.size(); k++) {
+    SceneHost& s = *scenes[k];
+    for (int w = s.dev.wave_begin; w < s.dev.wave_end; w++) hw[w] = (int)k;
+    for (int w = s.s_begin / 64; w < (s.s_begin + round_up64(s.ns)) / 64; w++) hsw[w] = (int)k;
+    hs[k] = s.dev;
+  }
+  if (!scenes.empty()) HIPCHK(hipMemcpyAsync((void*)nd.scenes, hs.data(), sizeof(SceneDev) * hs.size(), hipMemcpyHostToDevice, stream));
+  if (nd.W) HIPCHK(hipMemcpyAsync((void*)nd.wave_scene, hw.data(), sizeof(int) * nd.W, hipMemcpyHostToDevice, stream));
+  if (nd.SW) HIPCHK(hipMemcpyAsync((void*)nd.s_wave_scene, hsw.data(), sizeof(int) * nd.SW, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemsetAsync(nd.state, 0, sizeof(SceneState) * std::max<size_t>(scenes.size(), 1), stream));
+  launch_fill_i32(nd.info, P, -1, stream);
+  launch_fill_i32(nd.info_alt, P, -1, stream);
+  launch_fill_i32(nd.s_of, P, -1, stream);
+  launch_fill_i32(nd.s_pid, NS, -1, stream);
+  launch_fill_i32(nd.rest_cnt, NS, 0, stream);
+  launch_fill_i32(nd.nnbr, P, 0, stream);
+  launch_fill_f4(nd.pos, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.pos_alt, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.vel, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.vel_alt, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.x0, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  HIPCHK(hipMemsetAsync(nd.rho, 0, sizeof(float) * std::max(P, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.alpha, 0, sizeof(float) * std::max(P, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.src, 0, sizeof(float) * std::max(P, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.gam, 0, sizeof(float) * std::max(P, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.eta, 0, sizeof(float) * std::max(P, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.acc, 0, sizeof(float4) * std::max(P, 1), stream));
+  // copy persistent state of relocated scenes
+  for (size_t k = 0; k < scenes.size(); k++) {
+    if (!src[k].b) continue;
+    const SceneHost& s = *scenes[k];
+    const Dev& od = src[k].b->d;
+    const int ob = src[k].pb, nb = s.p_begin, n = s.n;
+    auto cp = [&](void* dst, const void* sp, size_t bytes) -> hipError_t {
+      return bytes ? hipMemcpyAsync(dst, sp, bytes, hipMemcpyDeviceToDevice, stream) : hipSuccess;
+    };
+    HIPCHK(cp(nd.pos + nb, od.pos + ob, sizeof(float4) * n));
+    HIPCHK(cp(nd.vel + nb, od.vel + ob, sizeof(float4) * n));
+    HIPCHK(cp(nd.info + nb, od.info + ob, sizeof(int) * n));
+    HIPCHK(cp(nd.inv + nb, od.inv + ob, sizeof(int) * n));
+    HIPCHK(cp(nd.x0 + nb, od.x0 + ob, sizeof(float4) * n));
+    HIPCHK(cp(nd.s_of + nb, od.s_of + ob, sizeof(int) * n));
+    HIPCHK(cp(nd.rho + nb, od.rho + ob, sizeof(float) * n));
+    HIPCHK(cp(nd.alpha + nb, od.alpha + ob, sizeof(float) * n));
+    HIPCHK(cp(nd.src + nb, od.src + ob, sizeof(float) * n));
+    HIPCHK(cp(nd.acc + nb, od.acc + ob, sizeof(float4) * n));
+    HIPCHK(cp(nd.gam + nb, od.gam + ob, sizeof(float) * n));
+    HIPCHK(cp(nd.eta + nb, od.eta + ob, sizeof(float) * n));
+    if (s.ns > 0) {
+      const int os = src[k].sb, ns_ = s.s_begin;
+      HIPCHK(cp(nd.s_pid + ns_, od.s_pid + os, sizeof(int) * s.ns));
+      HIPCHK(cp(nd.rest_cnt + ns_, od.rest_cnt + os, sizeof(int) * s.ns));
+      HIPCHK(cp(nd.rs + 4L * ns_, od.rs + 4L * os, sizeof(float4) * 4 * s.ns));
+      HIPCHK(hipMemcpy2DAsync(nd.rest + ns_, sizeof(int) * nd.NS, od.rest + os, sizeof(int) * od.NS,
+                              sizeof(int) * s.ns, MAX_REST, hipMemcpyDeviceToDevice, stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  // free the old arrays of this batch
+  for (auto& a : old_allocs) (void)hipFree(a.p);
+  d = nd;
+  return NN_OK;
+}
+
+int Batch::refresh_scene_table() {
+  std::vector<SceneDev> hs(scenes.size());
+  for (size_t k = 0; k < scenes.size(); k++) hs[k] = scenes[k]->dev;
+  if (!hs.empty())
+    HIPCHK(hipMemcpyAsync((void*)d.scenes, hs.data(), sizeof(SceneDev) * hs.size(), hipMemcpyHostToDevice, stream));
+  max_iter_[0] = max_iter_[1] = max_iter_[2] = 1;
+  for (auto& sp : scenes) {
+    max_iter_[0] = std::max(max_iter_[0], sp->dev.max_iter_v);
+    max_iter_[1] = std::max(max_iter_[1], sp->dev.max_iter_p);
+    max_iter_[2] = std::max(max_iter_[2], sp->dev.max_iter_cg);
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+int Batch::upload(int k, const float* x, const float* v, const int* kind) {
+  SceneHost& s = *scenes[k];
+  const int n = s.n, pb = s.p_begin;
+  const float V0 = (float)(0.8 * std::pow(2.0 * s.params.particle_radius, 3));  // :607, :741
+  std::vector<float4> hp(n), hv(n);
+  std::vector<int> hi(n), hinv(n);
+  for (int i = 0; i < n; i++) {
+    const int kd = kind[i];
+    hp[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], kd == KIND_FLUID ? V0 : -V0);
+    hv[i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    hi[i] = (kd << 28) | i;
+    hinv[i] = i;
+  }
+  HIPCHK(hipMemcpyAsync(d.pos + pb, hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(d.vel + pb, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(d.info + pb, hi.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(d.inv + pb, hinv.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  s.initialized = false;
+  s.stepped = false;
+  return NN_OK;
+}
+
+int Batch::search_impl() {
+  launch_fill_i32(d.cell_count, (long)C_total_ + 1, 0, stream);
+  launch_cell_count(d, block, stream);
+  launch_scan(d, C_total_ + 1, stream);
+  launch_bucket(d, block, stream);
+  launch_place(d, block, stream);
+  swap_sort_buffers();
+  launch_build_lists(d, block, stream);
+  return NN_OK;
+}
+
+int Batch::search() {
+  if (scenes.empty()) return NN_OK;
+  launch_step_begin(d, stream);
+  CHK(search_impl());
+  HIPCHK(hipGetLastError());
+  CHK(gather_state());
+  for (size_t k = 0; k < scenes.size(); k++) {
+    const int stt = h_state_[k].status;
+    if (stt & ST_OUT_OF_GRID) { set_error("particle outside the grid"); return NN_E_OUT_OF_GRID; }
+    if (stt & ST_CELL_OVERFLOW) { set_error("more than 50 particles in a cell"); return NN_E_CELL_OVERFLOW; }
+    if (stt & ST_NEIGHBOR_OVERFLOW) { set_error("more than 60 neighbors"); return NN_E_NEIGHBOR_OVERFLOW; }
+  }
+  return NN_OK;
+}
+
+int Batch::gather_state() {
+  const size_t ns = scenes.size();
+  if (ns > 4096) {
+    set_error("too many scenes in one batch");
+    return NN_E_ARG;
+  }
+  HIPCHK(hipMemcpyAsync(h_state_, d.state, sizeof(SceneState) * ns, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+// stress set: elastic particles and their rest neighbors (first search), with
+// rest lists capped at 50 (Elasticity.initValues_kernel, :1220-1236)
+int Batch::build_stress_set(const std::vector<int>& which) {
+  bool any = false;
+  for (int k : which) {
+    SceneHost& s = *scenes[k];
+    s.s_pid.clear();
+    s.rest.clear();
+    s.rest_cnt.clear();
+    s.ns = 0;
+    if (s.n_elastic == 0) continue;
+    any = true;
+    const int n = s.n, pb = s.p_begin;
+    std::vector<int> info(n), cnt(n), nb((size_t)n * MAX_NB);
+    HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), d.nnbr + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpy2DAsync(nb.data(), sizeof(int) * n, d.nbr + pb, sizeof(int) * d.P, sizeof(int) * n, MAX_NB,
+                            hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<int> slot_of_id(n, -1);
+    for (int t = 0; t < n; t++) slot_of_id[info[t] & 0x0FFFFFFF] = t;
+    std::vector<char> mark(n, 0);
+    for (int t = 0; t < n; t++) {
+      if ((info[t] >> 28) != KIND_ELASTIC) continue;
+      mark[info[t] & 0x0FFFFFFF] = 1;
+      for (int m = 0; m < cnt[t]; m++) mark[info[nb[(size_t)m * n + t] - pb] & 0x0FFFFFFF] = 1;
+    }
+    std::vector<int> s_of(n, -1);
+    for (int id = 0; id < n; id++)
+      if (mark[id]) {
+        s_of[id] = (int)s.s_pid.size();
+        s.s_pid.push_back(id);
+      }
+    s.ns = (int)s.s_pid.size();
+    s.rest.assign((size_t)s.ns * MAX_REST, -1);
+    s.rest_cnt.assign(s.ns, 0);
+    bool overflow = false;
+    for (int q = 0; q < s.ns; q++) {
+      const int t = slot_of_id[s.s_pid[q]];
+      int c = cnt[t];
+      if (c > MAX_REST) {
+        overflow = true;
+        c = MAX_REST;
+      }
+      s.rest_cnt[q] = c;
+      for (int m = 0; m < c; m++) s.rest[(size_t)q * MAX_REST + m] = info[nb[(size_t)m * n + t] - pb] & 0x0FFFFFFF;
+    }
+    if (overflow) {
+      set_error("an elastic particle has more than 50 rest neighbors");
+      return NN_E_REST_OVERFLOW;
+    }
+    s.dev.ns = s.ns;
+    s_of.swap(s.s_of_host);  // uploaded after the relayout below
+  }
+  if (!any) return NN_OK;
+  // re-layout with the new stress-set sizes, keeping all particle state
+  CHK(layout({this}));
+  for (int k : which) {
+    SceneHost& s = *scenes[k];
+    if (s.ns == 0) continue;
+    const int sb = s.s_begin, pb = s.p_begin;
+    std::vector<int> rest_t((size_t)s.ns * MAX_REST);
+    for (int q = 0; q < s.ns; q++)
+      for (int m = 0; m < MAX_REST; m++) rest_t[(size_t)m * s.ns + q] = s.rest[(size_t)q * MAX_REST + m];
+    HIPCHK(hipMemcpyAsync(d.s_pid + sb, s.s_pid.data(), sizeof(int) * s.ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d.rest_cnt + sb, s.rest_cnt.data(), sizeof(int) * s.ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpy2DAsync(d.rest + sb, sizeof(int) * d.NS, rest_t.data(), sizeof(int) * s.ns, sizeof(int) * s.ns,
+                            MAX_REST, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d.s_of + pb, s.s_of_host.data(), sizeof(int) * s.n, hipMemcpyHostToDevice, stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+int Batch::init_scenes(const std::vector<int>& which) {
+  if (scenes.empty()) return NN_OK;
+  launch_step_begin(d, stream);
+  CHK(search_impl());
+  launch_elastic_init(d, block, stream);
+  HIPCHK(hipGetLastError());
+  CHK(gather_state());
+  for (size_t k = 0; k < scenes.size(); k++) {
+    const int stt = h_state_[k].status;
+    if (stt & ST_OUT_OF_GRID) { set_error("particle outside the grid"); return NN_E_OUT_OF_GRID; }
+    if (stt & ST_CELL_OVERFLOW) { set_error("more than 50 particles in a cell"); return NN_E_CELL_OVERFLOW; }
+    if (stt & ST_NEIGHBOR_OVERFLOW) { set_error("more than 60 neighbors"); return NN_E_NEIGHBOR_OVERFLOW; }
+  }
+  CHK(build_stress_set(which));
+  launch_bvol(d, block, 0, stream);
+  launch_bvol(d, block, 1, stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  for (int k : which) scenes[k]->initialized = true;
+  return NN_OK;
+}
+
+// Runs solver iterations until no scene is active: the first chunk is the
+// previous step's count, later chunks two at a time; each chunk ends with one
+// 4-byte read of the active-scene counter.
+int Batch::solver_loop(int which) {
+  int pred = 1;
+  for (auto& sp : scenes) pred = std::max(pred, sp->last_iter[which]);
+  const int maxit = std::max(1, max_iter_[which]);
+  int done = 0, chunk = std::min(pred, maxit);
+  for (;;) {
+    for (int c = 0; c < chunk; c++) {
+      if (which == 0) {
+        launch_div_update(d, block, stream);
+        launch_div_source(d, block, stream);
+      } else if (which == 1) {
+        launch_press_update(d, block, stream);
+        launch_press_source(d, block, 1, stream);
+      } else {
+        launch_cg_matvec(d, block, stream);
+        launch_cg_update(d, block, stream);
+        std::swap(d.wp, d.wp_alt);
+      }
+    }
+    done += chunk;
+    if (done >= maxit) break;
+    HIPCHK(hipMemcpyAsync(h_active_, d.n_active + which, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (*h_active_ <= 0) break;
+    chunk = std::min(2, maxit - done);
+  }
+  return NN_OK;
+}
+
+int Batch::step(std::vector<nn_step_stats>* stats) {
+  if (scenes.empty()) return NN_OK;
+  for (auto& sp : scenes)
+    if (!sp->initialized) {
+      set_error("step before init");
+      return NN_E_STATE;
+    }
+  launch_step_begin(d, stream);
+  CHK(search_impl());                                  // :917
+  if (any_elastic_) launch_bvol(d, block, 1, stream);  // :918
+  launch_clear_acc(d, block, stream);                  // :920
+  if (any_fluid_) {                                    // DFSPHSolver.substep, :1546-1555
+    launch_density_alpha(d, block, stream);
+    if (any_div_) {
+      launch_solver_begin(d, 0, stream);
+      CHK(solver_loop(0));
+    }
+    if (any_strain_) launch_strain(d, block, stream);
+    if (any_weiler_) {
+      launch_solver_begin(d, 2, stream);
+      launch_cg_init(d, block, stream);
+      CHK(solver_loop(2));
+    }
+    launch_nonpressure(d, block, stream);
+    std::swap(d.vel, d.vel_alt);
+    launch_solver_begin(d, 1, stream);
+    launch_press_source(d, block, 0, stream);
+    CHK(solver_loop(1));
+  }
+  if (any_elastic_) {  // Elasticity.substep, :1241-1244
+    launch_elastic_rot_stress(d, block, stream);
+    launch_elastic_force(d, block, stream);
+  }
+  launch_advect(d, block, stream);  // :933, :936-937
+  HIPCHK(hipGetLastError());
+  CHK(gather_state());
+  int rc = NN_OK;
+  if (stats) stats->resize(scenes.size());
+  for (size_t k = 0; k < scenes.size(); k++) {
+    SceneHost& s = *scenes[k];
+    const SceneState& st = h_state_[k];
+    nn_step_stats o{};
+    o.iterations_v = st.iter_v;
+    o.iterations_p = st.iter_p;
+    o.iterations_cg = st.iter_cg;
+    o.status = st.status;
+    o.avg_density_err_v = st.err_v;
+    o.avg_density_err_p = st.err_p;
+    o.cg_rel_residual = st.cg_bb > 0.0 ? std::sqrt(st.cg_rr / st.cg_bb) : 0.0;
+    float mx;
+    unsigned bits = st.max_rho_bits;
+    std::memcpy(&mx, &bits, 4);
+    o.max_density = mx;
+    o.num_pairs = (int64_t)st.pairs;
+    s.last = o;
+    s.stepped = true;
+    s.last_iter[0] = std::max(1, st.iter_v);
+    s.last_iter[1] = std::max(1, st.iter_p);
+    s.last_iter[2] = std::max(1, st.iter_cg);
+    if (stats) (*stats)[k] = o;
+    int e = NN_OK;
+    if (st.status & ST_NAN) e = NN_E_NAN;
+    if (st.status & ST_NEIGHBOR_OVERFLOW) e = NN_E_NEIGHBOR_OVERFLOW;
+    if (st.status & ST_CELL_OVERFLOW) e = NN_E_CELL_OVERFLOW;
+    if (st.status & ST_OUT_OF_GRID) e = NN_E_OUT_OF_GRID;
+    if (e < 0 && (rc >= 0 || e < rc)) rc = e;
+    if (e == NN_OK && (st.status & ST_NOCONV) && rc == NN_OK) rc = NN_W_NOCONV;
+  }
+  if (rc < 0) set_error("step reported an error status (see nn_step_stats.status)");
+  return rc;
+}
+
+int Batch::synchronize() {
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// downloads in original order
+// ---------------------------------------------------------------------------
+static int field_width(int field) {
+  switch (field) {
+    case NN_FIELD_X:
+    case NN_FIELD_V:
+    case NN_FIELD_ACCELERATION:
+    case NN_FIELD_X0:
+      return 3;
+    case NN_FIELD_ROTATION:
+      return 9;
+    case NN_FIELD_STRESS:
+      return 6;
+    case NN_FIELD_VOLUME:
+    case NN_FIELD_DENSITY:
+    case NN_FIELD_DFSPH_FACTOR:
+    case NN_FIELD_DENSITY_ADV:
+    case NN_FIELD_STRAIN_NORM:
+    case NN_FIELD_VISCOSITY:
+    case NN_FIELD_REST_VOLUME:
+    case NN_FIELD_KIND:
+    case NN_FIELD_NUM_NEIGHBORS:
+      return 1;
+    default:
+      return 0;
+  }
+}
+
+int Batch::download(int k, int field, void* host, size_t bytes) {
+  SceneHost& s = *scenes[k];
+  const int w = field_width(field);
+  if (w == 0) {
+    set_error("unknown field");
+    return NN_E_ARG;
+  }
+  const int n = s.n, pb = s.p_begin;
+  if (bytes != (size_t)n * w * 4) {
+    set_error("buffer size does not match n * width * 4");
+    return NN_E_ARG;
+  }
+  std::vector<int> info(n);
+  HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+  float* out = (float*)host;
+  int* iout = (int*)host;
+  auto slot_f4 = [&](const float4* p, std::vector<float4>& v) -> hipError_t {
+    v.resize(n);
+    return hipMemcpyAsync(v.data(), p + pb, sizeof(float4) * n, hipMemcpyDeviceToHost, stream);
+  };
+  auto slot_f = [&](const float* p, std::vector<float>& v) -> hipError_t {
+    v.resize(n);
+    return hipMemcpyAsync(v.data(), p + pb, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
+  };
+  std::vector<float4> f4;
+  std::vector<float> f1;
+  std::vector<int> i1;
+  switch (field) {
+    case NN_FIELD_X:
+    case NN_FIELD_V:
+    case NN_FIELD_ACCELERATION:
+    case NN_FIELD_VOLUME: {
+      const float4* p = field == NN_FIELD_V ? d.vel : field == NN_FIELD_ACCELERATION ? d.acc : d.pos;
+      HIPCHK(slot_f4(p, f4));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int t = 0; t < n; t++) {
+        const int id = info[t] & 0x0FFFFFFF;
+        if (field == NN_FIELD_VOLUME) out[id] = std::fabs(f4[t].w);
+        else {
+          out[3 * id] = f4[t].x;
+          out[3 * id + 1] = f4[t].y;
+          out[3 * id + 2] = f4[t].z;
+        }
+      }
+      return NN_OK;
+    }
+    case NN_FIELD_X0:
+    case NN_FIELD_REST_VOLUME: {
+      f4.resize(n);
+      HIPCHK(hipMemcpyAsync(f4.data(), d.x0 + pb, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int id = 0; id < n; id++) {
+        if (field == NN_FIELD_REST_VOLUME) out[id] = f4[id].w;
+        else {
+          out[3 * id] = f4[id].x;
+          out[3 * id + 1] = f4[id].y;
+          out[3 * id + 2] = f4[id].z;
+        }
+      }
+      return NN_OK;
+    }
+    case NN_FIELD_DENSITY:
+    case NN_FIELD_DFSPH_FACTOR:
+    case NN_FIELD_DENSITY_ADV:
+    case NN_FIELD_STRAIN_NORM:
+    case NN_FIELD_VISCOSITY: {
+      const float* p = field == NN_FIELD_DENSITY ? d.rho
+                       : field == NN_FIELD_DFSPH_FACTOR ? d.alpha
+                       : field == NN_FIELD_DENSITY_ADV ? d.src
+                       : field == NN_FIELD_STRAIN_NORM ? d.gam
+                                                       : d.eta;
+      HIPCHK(slot_f(p, f1));
+      HIPCHK(hipStreamSynchronize(stream));
+      const SceneDev& c = s.dev;
+      for (int t = 0; t < n; t++) {
+        const int id = info[t] & 0x0FFFFFFF;
+        const bool fluid = (info[t] >> 28) == KIND_FLUID;
+        float val = f1[t];
+        if (field == NN_FIELD_DENSITY) val = (fluid && s.stepped) ? f1[t] : c.rho0;  // density.fill(density0), :744
+        else if (!fluid || !s.stepped) val = 0.0f;
+        else if (field == NN_FIELD_DFSPH_FACTOR) {
+          // the value the reference holds after a step: alpha * 1/dt * dt * 1/dt^2 (:1688,1713,1771)
+          val = (c.flags & SF_DIVERGENCE) ? ((f1[t] * c.inv_dt) * c.dt) * c.inv_dt2 : f1[t] * c.inv_dt2;
+        }
+        out[id] = val;
+      }
+      return NN_OK;
+    }
+    case NN_FIELD_KIND:
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int t = 0; t < n; t++) iout[info[t] & 0x0FFFFFFF] = info[t] >> 28;
+      return NN_OK;
+    case NN_FIELD_NUM_NEIGHBORS: {
+      i1.resize(n);
+      HIPCHK(hipMemcpyAsync(i1.data(), d.nnbr + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int t = 0; t < n; t++) iout[info[t] & 0x0FFFFFFF] = i1[t];
+      return NN_OK;
+    }
+    case NN_FIELD_ROTATION:
+    case NN_FIELD_STRESS: {
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int id = 0; id < n; id++) {
+        for (int a = 0; a < w; a++) out[(size_t)w * id + a] = (field == NN_FIELD_ROTATION && (a % 4) == 0) ? 1.0f : 0.0f;
+      }
+      if (s.ns > 0) {
+        std::vector<float4> rs((size_t)4 * s.ns);
+        HIPCHK(hipMemcpyAsync(rs.data(), d.rs + 4L * s.s_begin, sizeof(float4) * 4 * s.ns, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int q = 0; q < s.ns; q++) {
+          const int id = s.s_pid[q];
+          const float4* r = &rs[4 * (size_t)q];
+          if (field == NN_FIELD_ROTATION) {
+            const float R[9] = {r[0].x, r[0].y, r[0].z, r[1].x, r[1].y, r[1].z, r[2].x, r[2].y, r[2].z};
+            if (s.stepped) std::memcpy(out + 9 * (size_t)id, R, sizeof(R));
+          } else {
+            const float S[6] = {r[0].w, r[1].w, r[2].w, r[3].x, r[3].y, r[3].z};
+            if (s.stepped) std::memcpy(out + 6 * (size_t)id, S, sizeof(S));
+          }
+        }
+      }
+      return NN_OK;
+    }
+  }
+  set_error("unknown field");
+  return NN_E_ARG;
+}
+
+int Batch::upload_field(int k, int field, const void* host, size_t bytes) {
+  SceneHost& s = *scenes[k];
+  const int n = s.n, pb = s.p_begin;
+  if ((field != NN_FIELD_X && field != NN_FIELD_V) || bytes != (size_t)n * 12) {
+    set_error("only x and v (float[n][3]) can be uploaded");
+    return NN_E_ARG;
+  }
+  std::vector<int> info(n);
+  std::vector<float4> f4(n);
+  float4* p = field == NN_FIELD_X ? d.pos : d.vel;
+  HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(f4.data(), p + pb, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  const float* in = (const float*)host;
+  for (int t = 0; t < n; t++) {
+    const int id = info[t] & 0x0FFFFFFF;
+    f4[t].x = in[3 * id];
+    f4[t].y = in[3 * id + 1];
+    f4[t].z = in[3 * id + 2];
+  }
+  HIPCHK(hipMemcpyAsync(p + pb, f4.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+int Batch::neighbors(int k, int* nbr, int* cnt, int cap) {
+  SceneHost& s = *scenes[k];
+  const int n = s.n, pb = s.p_begin;
+  if (cap < MAX_NB) {
+    set_error("cap must be at least 60");
+    return NN_E_ARG;
+  }
+  std::vector<int> info(n), c(n), nb((size_t)n * MAX_NB);
+  HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(c.data(), d.nnbr + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpy2DAsync(nb.data(), sizeof(int) * n, d.nbr + pb, sizeof(int) * d.P, sizeof(int) * n, MAX_NB,
+                          hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  for (size_t t = 0; t < (size_t)n * cap; t++) nbr[t] = -1;
+  for (int t = 0; t < n; t++) {
+    const int id = info[t] & 0x0FFFFFFF;
+    cnt[id] = c[t];
+    for (int m = 0; m < c[t]; m++) nbr[(size_t)id * cap + m] = info[nb[(size_t)m * n + t] - pb] & 0x0FFFFFFF;
+  }
+  return NN_OK;
+}
+
+}  // namespace nn

@@ -1,0 +1,487 @@
+// capi.hip -- extern "C" entry points of libnnsph.so (declared in include/nnsph.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "batch.h"
+
+namespace nn {
+const char* last_error_text();
+}
+
+using nn::Batch;
+using nn::SceneHost;
+
+struct nn_ctx {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Batch> batch;
+  std::shared_ptr<SceneHost> scene;
+  int64_t max_particles = 0;
+  bool uploaded = false;
+};
+
+static int count_kinds(SceneHost& s, const int32_t* kind, int64_t n) {
+  s.n = (int)n;
+  s.n_fluid = s.n_elastic = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int k = kind[i];
+    if (k == NN_KIND_FLUID) s.n_fluid++;
+    else if (k == NN_KIND_ELASTIC) s.n_elastic++;
+    else if (k != NN_KIND_STATIC) {
+      nn::set_error("kind must be NN_KIND_FLUID, NN_KIND_STATIC or NN_KIND_ELASTIC (rigid bodies are not built)");
+      return k == 3 ? NN_E_UNSUPPORTED : NN_E_ARG;
+    }
+  }
+  return NN_OK;
+}
+
+
+extern "C" {
+
+int32_t nn_version(void) { return NNSPH_ABI_VERSION; }
+
+const char* nn_last_error(void) { return nn::last_error_text(); }
+
+int32_t nn_device_info(char* name, int32_t name_len, int64_t* hbm_bytes, int32_t* num_cus) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nn::hip_fail(e, "hipGetDevice");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return nn::hip_fail(e, "hipGetDeviceProperties");
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s", prop.gcnArchName);
+  }
+  if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+  if (num_cus) *num_cus = prop.multiProcessorCount;
+  return NN_OK;
+}
+
+int32_t nn_create(const nn_params* params, int64_t max_particles, nn_ctx** out) {
+  if (!params || !out || max_particles <= 0 || max_particles > (1LL << 28)) {
+    nn::set_error("nn_create: bad arguments");
+    return NN_E_ARG;
+  }
+  *out = nullptr;
+  auto ctx = new (std::nothrow) nn_ctx();
+  if (!ctx) return NN_E_OOM;
+  ctx->scene = std::make_shared<SceneHost>();
+  ctx->scene->params = *params;
+  std::string err;
+  int rc = nn::derive_scene(*ctx->scene, err);
+  if (rc < 0) {
+    nn::set_error(err);
+    delete ctx;
+    return rc;
+  }
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return nn::hip_fail(e, "hipStreamCreate");
+  }
+  ctx->batch.reset(new Batch(ctx->stream));
+  ctx->max_particles = max_particles;
+  *out = ctx;
+  return NN_OK;
+}
+
+int32_t nn_destroy(nn_ctx* ctx) {
+  if (!ctx) return NN_E_ARG;
+  ctx->batch.reset();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return NN_OK;
+}
+
+int32_t nn_upload(nn_ctx* ctx, const float* x, const float* v, const int32_t* kind, int64_t n) {
+  if (!ctx || !x || !kind || n <= 0) {
+    nn::set_error("nn_upload: bad arguments");
+    return NN_E_ARG;
+  }
+  if (n > ctx->max_particles) {
+    nn::set_error("nn_upload: more particles than max_particles");
+    return NN_E_ARG;
+  }
+  SceneHost& s = *ctx->scene;
+  int rc = count_kinds(s, kind, n);
+  if (rc < 0) return rc;
+  std::string err;
+  rc = nn::derive_scene(s, err);
+  if (rc < 0) {
+    nn::set_error(err);
+    return rc;
+  }
+  s.ns = 0;
+  s.s_pid.clear();
+  s.rest.clear();
+  s.rest_cnt.clear();
+  ctx->batch->scenes = {ctx->scene};
+  rc = ctx->batch->layout({});
+  if (rc < 0) return rc;
+  rc = ctx->batch->upload(0, x, v, kind);
+  if (rc < 0) return rc;
+  ctx->uploaded = true;
+  return NN_OK;
+}
+
+int32_t nn_init(nn_ctx* ctx) {
+  if (!ctx || !ctx->uploaded) {
+    nn::set_error("nn_init before nn_upload");
+    return NN_E_STATE;
+  }
+  return ctx->batch->init_scenes({0});
+}
+
+int32_t nn_step(nn_ctx* ctx, int32_t nsteps, nn_step_stats* out) {
+  if (!ctx || !ctx->uploaded) {
+    nn::set_error("nn_step before nn_upload/nn_init");
+    return NN_E_STATE;
+  }
+  int rc = NN_OK;
+  std::vector<nn_step_stats> st;
+  for (int k = 0; k < nsteps; k++) {
+    rc = ctx->batch->step(&st);
+    if (rc < 0) break;
+  }
+  if (out && !st.empty()) *out = st[0];
+  return rc;
+}
+
+int32_t nn_search(nn_ctx* ctx) {
+  if (!ctx || !ctx->uploaded) return NN_E_STATE;
+  return ctx->batch->search();
+}
+
+int32_t nn_download(nn_ctx* ctx, int32_t field, void* host, size_t bytes) {
+  if (!ctx || !ctx->uploaded || !host) return NN_E_STATE;
+  return ctx->batch->download(0, field, host, bytes);
+}
+
+int32_t nn_upload_field(nn_ctx* ctx, int32_t field, const void* host, size_t bytes) {
+  if (!ctx || !ctx->uploaded || !host) return NN_E_STATE;
+  return ctx->batch->upload_field(0, field, host, bytes);
+}
+
+int32_t nn_neighbors(nn_ctx* ctx, int32_t* nbr, int32_t* count, int32_t cap) {
+  if (!ctx || !ctx->uploaded || !nbr || !count) return NN_E_STATE;
+  return ctx->batch->neighbors(0, nbr, count, cap);
+}
+
+static int apply_param(SceneHost& s, const char* key, double value) {
+  nn_params& p = s.params;
+  if (!key) return NN_E_ARG;
+  const std::string k(key);
+  if (k == "dt") p.dt = value;
+  else if (k == "viscosity") p.viscosity = value;
+  else if (k == "boundary_viscosity") p.boundary_viscosity = value;
+  else if (k == "max_error") p.max_error = value;
+  else if (k == "max_error_v") p.max_error_v = value;
+  else if (k == "surface_tension") p.surface_tension = value;
+  else if (k == "max_iterations") p.max_iterations = (int32_t)value;
+  else if (k == "max_iterations_v") p.max_iterations_v = (int32_t)value;
+  else if (k == "weiler_tol") p.weiler_tol = value;
+  else if (k == "weiler_max_iter") p.weiler_max_iter = (int32_t)value;
+  else {
+    nn::set_error("nn_set_param: unknown or immutable key " + k);
+    return NN_E_ARG;
+  }
+  std::string err;
+  const nn::SceneDev keep = s.dev;
+  int rc = nn::derive_scene(s, err);
+  if (rc < 0) {
+    nn::set_error(err);
+    return rc;
+  }
+  // layout fields are not derived from the parameters
+  s.dev.p_begin = keep.p_begin;
+  s.dev.shift = keep.shift;
+  s.dev.wave_begin = keep.wave_begin;
+  s.dev.wave_end = keep.wave_end;
+  s.dev.cell_begin = keep.cell_begin;
+  s.dev.s_begin = keep.s_begin;
+  s.dev.ns = keep.ns;
+  return NN_OK;
+}
+
+int32_t nn_set_param(nn_ctx* ctx, const char* key, double value) {
+  if (!ctx) return NN_E_ARG;
+  int rc = apply_param(*ctx->scene, key, value);
+  if (rc < 0) return rc;
+  if (ctx->uploaded) return ctx->batch->refresh_scene_table();
+  return NN_OK;
+}
+
+int32_t nn_counts(nn_ctx* ctx, int64_t* n, int64_t* n_fluid) {
+  if (!ctx) return NN_E_ARG;
+  if (n) *n = ctx->scene->n;
+  if (n_fluid) *n_fluid = ctx->scene->n_fluid;
+  return NN_OK;
+}
+
+int32_t nn_set_block(nn_ctx* ctx, int32_t block) {
+  if (!ctx) return NN_E_ARG;
+  if (block == 0) block = 256;
+  if (block < 64 || block > 1024 || (block & (block - 1))) return NN_E_ARG;
+  ctx->batch->block = block;
+  return NN_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// pool: many resident scenes stepped together
+// ---------------------------------------------------------------------------
+static constexpr int POOL_CAPACITY = 4096;
+
+struct PoolSlot {
+  std::shared_ptr<SceneHost> scene;
+  std::unique_ptr<Batch> staging;  // set while the scene waits to join the batch
+  hipStream_t staging_stream = nullptr;
+  long long bytes = 0;
+  bool used = false, retired = false;
+};
+
+struct nn_pool {
+  hipStream_t stream = nullptr;
+  int64_t budget = 0;
+  long long in_use = 0;
+  std::unique_ptr<Batch> main;
+  std::vector<PoolSlot> slots;
+  std::mutex mu;
+  int block = 256;
+  std::vector<int> member_slot;  // batch index -> slot
+};
+
+extern "C" {
+
+int32_t nn_pool_capacity(void) { return POOL_CAPACITY; }
+
+int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) {
+  if (!out || budget_bytes <= 0) return NN_E_ARG;
+  auto p = new (std::nothrow) nn_pool();
+  if (!p) return NN_E_OOM;
+  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete p;
+    return nn::hip_fail(e, "hipStreamCreate");
+  }
+  p->budget = budget_bytes;
+  p->main.reset(new Batch(p->stream));
+  p->slots.resize(POOL_CAPACITY);
+  *out = p;
+  return NN_OK;
+}
+
+int32_t nn_pool_destroy(nn_pool* p) {
+  if (!p) return NN_E_ARG;
+  p->main.reset();
+  for (auto& s : p->slots) {
+    s.staging.reset();
+    if (s.staging_stream) (void)hipStreamDestroy(s.staging_stream);
+  }
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return NN_OK;
+}
+
+int64_t nn_pool_footprint(int64_t n, int64_t n_elastic, const nn_params* params) {
+  if (!params || n <= 0) return -1;
+  SceneHost s;
+  s.params = *params;
+  s.n = (int)n;
+  std::string err;
+  if (nn::derive_scene(s, err) < 0) return -1;
+  // the stress set is the elastic particles plus a one-particle-wide shell
+  const long long ns = n_elastic > 0 ? n_elastic + n_elastic / 2 + 64 : 0;
+  return nn::scene_footprint(n, ns, s.ncells);
+}
+
+int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const float* v, const int32_t* kind,
+                      int64_t n, int32_t* slot) {
+  if (!p || !params || !x || !kind || n <= 0 || !slot) return NN_E_ARG;
+  auto sc = std::make_shared<SceneHost>();
+  sc->params = *params;
+  int rc = count_kinds(*sc, kind, n);
+  if (rc < 0) return rc;
+  std::string err;
+  rc = nn::derive_scene(*sc, err);
+  if (rc < 0) {
+    nn::set_error(err);
+    return rc;
+  }
+  const long long ns_est = sc->n_elastic > 0 ? sc->n_elastic + sc->n_elastic / 2 + 64 : 0;
+  const long long fp = nn::scene_footprint(n, ns_est, sc->ncells);
+  int s_id = -1;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->in_use + 2 * fp > p->budget) {  // the scene lives twice while it is merged
+      nn::set_error("pool budget exceeded");
+      return NN_E_BUSY;
+    }
+    for (int k = 0; k < POOL_CAPACITY; k++)
+      if (!p->slots[k].used) {
+        s_id = k;
+        break;
+      }
+    if (s_id < 0) {
+      nn::set_error("pool has no free slot");
+      return NN_E_BUSY;
+    }
+    p->slots[s_id].used = true;
+    p->slots[s_id].retired = false;
+    p->slots[s_id].bytes = fp;
+    p->in_use += fp;
+  }
+  PoolSlot& ps = p->slots[s_id];
+  auto fail = [&](int code) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    ps.staging.reset();
+    ps.scene.reset();
+    ps.used = false;
+    p->in_use -= fp;
+    return code;
+  };
+  if (!ps.staging_stream) {
+    hipError_t e = hipStreamCreateWithFlags(&ps.staging_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(nn::hip_fail(e, "hipStreamCreate"));
+  }
+  std::unique_ptr<Batch> stg(new Batch(ps.staging_stream));
+  stg->block = p->block;
+  stg->scenes = {sc};
+  rc = stg->layout({});
+  if (rc < 0) return fail(rc);
+  rc = stg->upload(0, x, v, kind);
+  if (rc < 0) return fail(rc);
+  rc = stg->init_scenes({0});
+  if (rc < 0) return fail(rc);
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    ps.scene = sc;
+    ps.staging = std::move(stg);
+  }
+  *slot = s_id;
+  return NN_OK;
+}
+
+int32_t nn_pool_retire(nn_pool* p, int32_t slot) {
+  if (!p || slot < 0 || slot >= POOL_CAPACITY) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (!p->slots[slot].used) return NN_E_ARG;
+  p->slots[slot].retired = true;
+  return NN_OK;
+}
+
+// membership changes are applied at tick boundaries: retired scenes leave,
+// staged scenes join, the batch is re-laid out with state copied over
+static int pool_apply_membership(nn_pool* p) {
+  std::vector<std::shared_ptr<SceneHost>> members;
+  std::vector<int> mslot;
+  std::vector<const Batch*> from{p->main.get()};
+  bool changed = false;
+  for (int k = 0; k < POOL_CAPACITY; k++) {
+    PoolSlot& s = p->slots[k];
+    if (!s.used) continue;
+    if (s.retired) {
+      changed = true;
+      continue;
+    }
+    if (!s.scene) continue;  // admission still running
+    if (s.staging) {
+      changed = true;
+      from.push_back(s.staging.get());
+    }
+    members.push_back(s.scene);
+    mslot.push_back(k);
+  }
+  if (!changed) return NN_OK;
+  p->main->block = p->block;
+  p->main->scenes = members;
+  int rc = p->main->layout(from);
+  if (rc < 0) return rc;
+  for (int k = 0; k < POOL_CAPACITY; k++) {
+    PoolSlot& s = p->slots[k];
+    if (!s.used) continue;
+    if (s.retired) {
+      s.staging.reset();
+      s.scene.reset();
+      s.used = false;
+      s.retired = false;
+      p->in_use -= s.bytes;
+      s.bytes = 0;
+      continue;
+    }
+    if (s.staging && s.scene) s.staging.reset();
+  }
+  p->member_slot = mslot;
+  return NN_OK;
+}
+
+int32_t nn_pool_tick(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_t* stepped) {
+  if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = pool_apply_membership(p);
+  if (rc < 0) return rc;
+  if (stepped) *stepped = (int32_t)p->main->scenes.size();
+  if (p->main->scenes.empty()) return NN_OK;
+  std::vector<nn_step_stats> st;
+  rc = p->main->step(&st);
+  if (stats) {
+    for (size_t k = 0; k < st.size(); k++) {
+      const int s = p->member_slot[k];
+      if (s < stats_len) stats[s] = st[k];
+    }
+  }
+  return rc;
+}
+
+static Batch* pool_locate(nn_pool* p, int slot, int& index) {
+  PoolSlot& s = p->slots[slot];
+  if (!s.used || !s.scene) return nullptr;
+  Batch* b = s.staging ? s.staging.get() : p->main.get();
+  for (size_t k = 0; k < b->scenes.size(); k++)
+    if (b->scenes[k].get() == s.scene.get()) {
+      index = (int)k;
+      return b;
+    }
+  return nullptr;
+}
+
+int32_t nn_pool_read(nn_pool* p, int32_t slot, int32_t field, void* host, size_t bytes) {
+  if (!p || slot < 0 || slot >= POOL_CAPACITY || !host) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  int idx = 0;
+  Batch* b = pool_locate(p, slot, idx);
+  if (!b) {
+    nn::set_error("slot not resident");
+    return NN_E_ARG;
+  }
+  return b->download(idx, field, host, bytes);
+}
+
+int32_t nn_pool_info(nn_pool* p, int32_t* resident, int32_t* pending, int64_t* bytes_in_use) {
+  if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  int r = 0, q = 0;
+  for (auto& s : p->slots) {
+    if (!s.used || !s.scene) continue;
+    if (s.staging) q++;
+    else r++;
+  }
+  if (resident) *resident = r;
+  if (pending) *pending = q;
+  if (bytes_in_use) *bytes_in_use = p->in_use;
+  return NN_OK;
+}
+
+int32_t nn_pool_set_block(nn_pool* p, int32_t block) {
+  if (!p) return NN_E_ARG;
+  if (block == 0) block = 256;
+  if (block < 64 || block > 1024 || (block & (block - 1))) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->block = block;
+  p->main->block = block;
+  return NN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// engine.h -- device data layout and kernel interfaces of libnnsph.
+//
+// Layout in HBM (one concatenated, offset-indexed array for all scenes of a batch):
+//  * Particle slots [0, P): scene s owns [p_begin, p_begin + n), p_begin a multiple
+//    of 64 so every wavefront belongs to exactly one scene; slots past n up to the
+//    next multiple of 64 are padding.  Inside a scene particles are kept in cell
+//    order (cell key, then original index), re-sorted at every neighbor search.
+//  * pos = float4 (x, y, z, signed volume): +V for fluid, -V for solids, so one
+//    16-byte gather per neighbor yields position, volume and material.
+//  * vel = float4 (vx, vy, vz, 0); acc = float4.
+//  * info = kind << 28 | original index (scene-local); -1 on padding slots.
+//  * Neighbor lists: nbr[k * P + i], k < 60, so lane i and lane i+1 read adjacent
+//    words; nnbr[i] holds the count.
+//  * Original-index ("id") space: inv[p_begin + id] = local slot; x0 (rest
+//    position, w = rest volume) and the stress-set map live there.
+//  * Cells: scene s owns global cell keys [cell_begin, cell_begin + gn0*gn1*gn2],
+//    key = cell_begin + (cx * gn1 + cy) * gn2 + cz; the extra last key parks
+//    particles that left the grid (reported as an error, never searched).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nn {
+
+constexpr int MAX_NB = 60;    // NeighborhoodSearchSparse.max_num_neighbors, SPH/main.py:777
+constexpr int MAX_PIC = 50;   // max_num_particles_in_grid, SPH/main.py:778
+constexpr int MAX_REST = 50;  // Elasticity.max_num_neighbors, SPH/main.py:1192
+constexpr int WAVE = 64;
+
+enum { KIND_FLUID = 0, KIND_STATIC = 1, KIND_ELASTIC = 2 };
+enum {
+  ST_NEIGHBOR_OVERFLOW = 1,
+  ST_CELL_OVERFLOW = 2,
+  ST_OUT_OF_GRID = 4,
+  ST_NAN = 8,
+  ST_REST_OVERFLOW = 16,
+  ST_NOCONV = 32
+};
+enum {
+  SF_SURFACE_TENSION = 1,
+  SF_VISCOSITY = 2,
+  SF_NON_NEWTONIAN = 4,
+  SF_WEILER = 8,
+  SF_DIVERGENCE = 16,
+  SF_COMPENSATED = 32,
+  SF_HAS_FLUID = 64,
+  SF_HAS_ELASTIC = 128
+};
+
+// Per-scene constants (device table).  Floats are the reference's FP32
+// constants, derived in double on the host and rounded once.
+struct SceneDev {
+  int p_begin, n, n_fluid, n_elastic;
+  int cell_begin, gn0, gn1, gn2;
+  int shift;  // p_begin minus the unpadded particle prefix
+  int wave_begin, wave_end;
+  int s_begin, ns;  // stress-set range
+  int flags, nn_method;
+  int max_iter_v, max_iter_p, max_iter_cg;
+  float h, thr2, kW, kG, W0;
+  float dt, inv_dt, inv_dt2, rho0, mass;
+  float nu, bnu, gx, gy, gz;
+  float lo0, lo1, lo2, hi0, hi1, hi2;
+  float diam2, Wd, visc_eps, sigma, opc, sticky, eps;
+  float C_diag, C_off, C_shear;
+  float nn_n, nn_K, nn_e0, nn_einf, nn_gc, nn_muc, nn_tauy, nn_max;
+  float ten_nu, ten_bnu, sig_m;  // 10*nu, 10*bnu, (sigma/m)*m
+  float pad0;
+  double eta_v, eta_p, cg_tol2, nf_d, rho0_d;
+};
+
+// Per-scene mutable solver state.
+struct SceneState {
+  int active_v, active_p, active_cg, pad_a;
+  int iter_v, iter_p, iter_cg, status;
+  unsigned counter, max_rho_bits;
+  unsigned long long pairs;
+  double err_v, err_p;
+  double cg_bb, cg_rr, cg_rz, cg_alpha, cg_beta, cg_pq;
+};
+
+// Device pointers of a batch (passed by value to every kernel).
+struct Dev {
+  int P;        // particle slots (multiple of 64)
+  int W;        // waves = P / 64
+  int C;        // total cells
+  int NS;       // stress-set slots (each scene's range padded to 64)
+  int SW;       // stress-set waves = NS / 64
+  int nscenes;
+  const SceneDev* scenes;
+  SceneState* state;
+  const int* wave_scene;
+  const int* s_wave_scene;  // stress-set wave -> scene
+  float4* pos;  // current
+  float4* pos_alt;
+  float4* vel;  // current
+  float4* vel_alt;
+  int* info;
+  int* info_alt;
+  float4* acc;
+  float* rho;
+  float* alpha;
+  float* src;  // density_adv of the reference (rho-dot in the divergence solve)
+  int* nbr;
+  int* nnbr;
+  int* key;
+  int* rank;
+  int* inv;
+  int* bucket;
+  int* cell_count;
+  int* cell_start;
+  int* scan_tmp;
+  double* partials;  // 3 * W
+  int* n_active;     // [0] divergence, [1] pressure, [2] cg
+  // Weiler CG
+  float4* wx;
+  float4* wr;
+  float4* wz;
+  float4* wp;
+  float4* wp_alt;
+  float4* wq;
+  float4* wd0;  // inverse diagonal block: (d00, d01, d02, d11)
+  float2* wd1;  //                          (d12, d22)
+  float* mu;    // dynamic viscosity per particle
+  float* gam;   // strain-rate norm
+  float* eta;   // kinematic viscosity from the model
+  // elasticity (id space and stress set)
+  float4* x0;   // rest position, w = rest volume
+  int* s_of;    // id space -> stress-set index (scene-local) or -1
+  int* s_pid;   // stress-set -> local id (-1 on padding)
+  int* rest;    // rest[k * NS + q]: local id of the k-th rest neighbor
+  int* rest_cnt;
+  float4* rs;   // 4 per member: rows of R with stress 0..2 in w, then (s3, s4, s5, 0)
+};
+
+// ---------------------------------------------------------------------------
+// host-side launchers (kernels_*.hip)
+// ---------------------------------------------------------------------------
+void launch_fill_i32(int* p, long n, int v, hipStream_t st);
+void launch_fill_f4(float4* p, long n, float4 v, hipStream_t st);
+void launch_scan(const Dev& d, int n, hipStream_t st);  // cell_start = exclusive scan(cell_count)
+
+void launch_cell_count(const Dev& d, int block, hipStream_t st);
+void launch_bucket(const Dev& d, int block, hipStream_t st);
+void launch_place(const Dev& d, int block, hipStream_t st);
+void launch_build_lists(const Dev& d, int block, hipStream_t st);
+void launch_bvol(const Dev& d, int block, int dynamic, hipStream_t st);
+
+void launch_density_alpha(const Dev& d, int block, hipStream_t st);
+void launch_div_update(const Dev& d, int block, hipStream_t st);
+void launch_div_source(const Dev& d, int block, hipStream_t st);
+void launch_solver_begin(const Dev& d, int which, hipStream_t st);
+void launch_strain(const Dev& d, int block, hipStream_t st);
+void launch_cg_init(const Dev& d, int block, hipStream_t st);
+void launch_cg_matvec(const Dev& d, int block, hipStream_t st);
+void launch_cg_update(const Dev& d, int block, hipStream_t st);
+void launch_nonpressure(const Dev& d, int block, hipStream_t st);
+void launch_press_source(const Dev& d, int block, int reduce, hipStream_t st);
+void launch_press_update(const Dev& d, int block, hipStream_t st);
+void launch_clear_acc(const Dev& d, int block, hipStream_t st);
+void launch_elastic_init(const Dev& d, int block, hipStream_t st);
+void launch_elastic_rot_stress(const Dev& d, int block, hipStream_t st);
+void launch_elastic_force(const Dev& d, int block, hipStream_t st);
+void launch_advect(const Dev& d, int block, hipStream_t st);
+void launch_step_begin(const Dev& d, hipStream_t st);
+
+}  // namespace nn

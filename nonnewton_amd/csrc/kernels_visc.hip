@@ -1,0 +1,273 @@
+// kernels_visc.hip -- non-Newtonian viscosity and the Weiler 2018 implicit
+// viscosity solver.
+//
+// The reference computes a per-particle non-Newtonian viscosity that no force
+// reads (NonNewton, SPH/main.py:332-492) and declares an implicit solver whose
+// operator is empty (Viscosity_Weiler2018, :1471-1513).  Here the strain rate
+// feeds the model, the model feeds the implicit solve:
+//   D_i   = (0.5 / rho_i) sum_j m_j (v_ji (x) gradW_ij + gradW_ij (x) v_ji)
+//   eta_i = model(||D_i||_F)                        (kinematic, :404-459)
+//   mu_i  = rho0 * eta_i   (rho0 * viscosity when the model is off)
+//   (A x)_i = x_i - (dt / rho_i) sum_{fluid j} c_ij ((x_i - x_j) . r_ij) gradW_ij
+//   c_ij  = 10 mu_ij (m / rho_j) / (|r_ij|^2 + 0.01 h^2),  mu_ij = (mu_i + mu_j) / 2
+// Weiler et al. 2018, "A Physically Consistent Implicit Viscosity Solver for
+// SPH Fluids", solved by preconditioned conjugate gradients: block-Jacobi 3x3
+// preconditioner, FP64 dot products, stop when ||r|| <= tol ||b|| with b = v.
+// CG iteration = two sweeps: matvec (with the search-direction update
+// p = z + beta p_old fused in, neighbors' p formed on the fly) and update.
+#include "device_common.h"
+
+namespace nn {
+
+__device__ __forceinline__ bool weiler_scene(const SceneDev& sc) {
+  return (sc.flags & SF_HAS_FLUID) && (sc.flags & SF_VISCOSITY) && (sc.flags & SF_WEILER);
+}
+
+// NonNewton.calcStrainRate (:461-492, with the accumulation the reference
+// omits) + the viscosity model; mu for the implicit solver.
+__global__ void k_strain(Dev d) {
+  int i, s;
+  if (!slot_scene(d, i, s)) return;
+  const SceneDev& sc = d.scenes[s];
+  if (!(sc.flags & SF_HAS_FLUID)) return;
+  const bool nn_on = (sc.flags & SF_NON_NEWTONIAN) != 0;
+  const bool weiler = weiler_scene(sc);
+  if (!nn_on && !weiler) return;
+  if (i >= sc.p_begin + sc.n || kind_of(d.info[i]) != KIND_FLUID) return;
+  float eta = 0.0f;
+  if (nn_on) {
+    const float4 xi = d.pos[i];
+    const float4 vi = d.vel[i];
+    const int cnt = d.nnbr[i];
+    const int P = d.P;
+    float S[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) S[q] = 0.0f;
+    for (int k = 0; k < cnt; k++) {
+      const int j = d.nbr[(long)k * P + i];
+      const float4 xj = d.pos[j];
+      const float4 vj = d.vel[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float3 g3 = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+      const float g[3] = {g3.x, g3.y, g3.z};
+      const float vji[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S[3 * a + b] += sc.mass * (vji[a] * g[b] + vji[b] * g[a]);
+    }
+    const float f = 0.5f / d.rho[i];
+    float n2 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const float t = f * S[q];
+      n2 += t * t;
+    }
+    const float gam = sqrtf(n2);
+    eta = nn_model(sc, gam);
+    d.gam[i] = gam;
+    d.eta[i] = eta;
+  }
+  if (weiler) d.mu[i] = (nn_on ? eta : sc.nu) * sc.rho0;
+}
+
+// c_ij for a fluid pair (i, j) and its gradient; returns false if j is not fluid
+__device__ __forceinline__ float weiler_coef(const SceneDev& sc, float mui, float muj, float rhoj, float rx, float ry,
+                                             float rz) {
+  const float muij = 0.5f * (mui + muj);
+  return 10.0f * muij * (sc.mass / rhoj) / ((rx * rx + ry * ry + rz * rz) + sc.visc_eps);
+}
+
+// r = b - A x0 with x0 = b = v, the inverse 3x3 diagonal block, z = D^-1 r,
+// p_old = 0; reduces (b.b, r.r, r.z) and activates the scene's CG.
+__global__ void k_cg_init(Dev d) {
+  int i, s;
+  if (!slot_scene(d, i, s)) return;
+  const SceneDev& sc = d.scenes[s];
+  if (!weiler_scene(sc)) return;
+  SceneState* st = d.state + s;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
+    const float4 xi = d.pos[i];
+    const float4 vi = d.vel[i];
+    const float mui = d.mu[i];
+    const int cnt = d.nnbr[i];
+    const int P = d.P;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
+    for (int k = 0; k < cnt; k++) {
+      const int j = d.nbr[(long)k * P + i];
+      const float4 xj = d.pos[j];
+      if (!(xj.w > 0.0f)) continue;
+      const float4 vj = d.vel[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+      const float c = weiler_coef(sc, mui, d.mu[j], d.rho[j], rx, ry, rz);
+      const float pr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
+      ax += c * pr * g.x;
+      ay += c * pr * g.y;
+      az += c * pr * g.z;
+      m0 += c * g.x * rx;
+      m1 += c * g.x * ry;
+      m2 += c * g.x * rz;
+      m3 += c * g.y * ry;
+      m4 += c * g.y * rz;
+      m5 += c * g.z * rz;
+    }
+    const float f = sc.dt / d.rho[i];
+    const float qx = vi.x - f * ax, qy = vi.y - f * ay, qz = vi.z - f * az;
+    const float rrx = vi.x - qx, rry = vi.y - qy, rrz = vi.z - qz;
+    const float d00 = 1.0f - f * m0, d01 = -f * m1, d02 = -f * m2;
+    const float d11 = 1.0f - f * m3, d12 = -f * m4, d22 = 1.0f - f * m5;
+    const float c00 = d11 * d22 - d12 * d12, c01 = d02 * d12 - d01 * d22, c02 = d01 * d12 - d02 * d11;
+    const float c11 = d00 * d22 - d02 * d02, c12 = d01 * d02 - d00 * d12, c22 = d00 * d11 - d01 * d01;
+    const float det = d00 * c00 + d01 * c01 + d02 * c02;
+    float4 w0;
+    float2 w1;
+    if (det > 0.0f && det == det) {
+      const float id = 1.0f / det;
+      w0 = make_float4(c00 * id, c01 * id, c02 * id, c11 * id);
+      w1 = make_float2(c12 * id, c22 * id);
+    } else {
+      w0 = make_float4(1.f, 0.f, 0.f, 1.f);
+      w1 = make_float2(0.f, 1.f);
+    }
+    const float zx = w0.x * rrx + w0.y * rry + w0.z * rrz;
+    const float zy = w0.y * rrx + w0.w * rry + w1.x * rrz;
+    const float zz = w0.z * rrx + w1.x * rry + w1.y * rrz;
+    d.wd0[i] = w0;
+    d.wd1[i] = w1;
+    d.wx[i] = make_float4(vi.x, vi.y, vi.z, 0.f);
+    d.wr[i] = make_float4(rrx, rry, rrz, 0.f);
+    d.wz[i] = make_float4(zx, zy, zz, 0.f);
+    v[0] = (double)vi.x * (double)vi.x + (double)vi.y * (double)vi.y + (double)vi.z * (double)vi.z;
+    v[1] = (double)rrx * (double)rrx + (double)rry * (double)rry + (double)rrz * (double)rrz;
+    v[2] = (double)rrx * (double)zx + (double)rry * (double)zy + (double)rrz * (double)zz;
+  }
+  double tot[3];
+  if (scene_reduce<3>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
+    st->cg_bb = tot[0];
+    st->cg_rr = tot[1];
+    st->cg_rz = tot[2];
+    st->iter_cg = 0;
+    st->cg_beta = 0.0;
+    const int act = (tot[0] > 0.0 && tot[1] > sc.cg_tol2 * tot[0]) ? 1 : 0;
+    st->active_cg = act;
+    if (act) atomicAdd(&d.n_active[2], 1);
+  }
+}
+
+__device__ __forceinline__ float4 cg_dir(const Dev& d, int j, int first, float beta) {
+  const float4 z = d.wz[j];
+  if (first) return z;
+  const float4 po = d.wp_alt[j];
+  return make_float4(z.x + beta * po.x, z.y + beta * po.y, z.z + beta * po.z, 0.f);
+}
+
+// p = z + beta p_old (p = z on the first iteration); q = A p; reduce p.q.
+__global__ void k_cg_matvec(Dev d) {
+  int i, s;
+  if (!slot_scene(d, i, s)) return;
+  const SceneDev& sc = d.scenes[s];
+  SceneState* st = d.state + s;
+  if (!weiler_scene(sc) || !st->active_cg) return;
+  const int first = st->iter_cg == 0;
+  const float beta = (float)st->cg_beta;
+  double v[1] = {0.0};
+  if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
+    const float4 xi = d.pos[i];
+    const float4 pi = cg_dir(d, i, first, beta);
+    const float mui = d.mu[i];
+    const int cnt = d.nnbr[i];
+    const int P = d.P;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int k = 0; k < cnt; k++) {
+      const int j = d.nbr[(long)k * P + i];
+      const float4 xj = d.pos[j];
+      if (!(xj.w > 0.0f)) continue;
+      const float4 pj = cg_dir(d, j, first, beta);
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+      const float c = weiler_coef(sc, mui, d.mu[j], d.rho[j], rx, ry, rz);
+      const float pr = (pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz;
+      ax += c * pr * g.x;
+      ay += c * pr * g.y;
+      az += c * pr * g.z;
+    }
+    const float f = sc.dt / d.rho[i];
+    const float qx = pi.x - f * ax, qy = pi.y - f * ay, qz = pi.z - f * az;
+    d.wp[i] = pi;
+    d.wq[i] = make_float4(qx, qy, qz, 0.f);
+    v[0] = (double)pi.x * (double)qx + (double)pi.y * (double)qy + (double)pi.z * (double)qz;
+  }
+  double tot[1];
+  if (scene_reduce<1>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
+    st->cg_pq = tot[0];
+    st->cg_alpha = st->cg_rz / tot[0];
+  }
+}
+
+// x += alpha p; r -= alpha q; z = D^-1 r; reduce (r.r, r.z); convergence test.
+__global__ void k_cg_update(Dev d) {
+  int i, s;
+  if (!slot_scene(d, i, s)) return;
+  const SceneDev& sc = d.scenes[s];
+  SceneState* st = d.state + s;
+  if (!weiler_scene(sc) || !st->active_cg) return;
+  const float al = (float)st->cg_alpha;
+  double v[2] = {0.0, 0.0};
+  if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
+    const float4 p = d.wp[i];
+    const float4 q = d.wq[i];
+    float4 x = d.wx[i];
+    float4 r = d.wr[i];
+    x.x += al * p.x;
+    x.y += al * p.y;
+    x.z += al * p.z;
+    r.x -= al * q.x;
+    r.y -= al * q.y;
+    r.z -= al * q.z;
+    const float4 w0 = d.wd0[i];
+    const float2 w1 = d.wd1[i];
+    const float zx = w0.x * r.x + w0.y * r.y + w0.z * r.z;
+    const float zy = w0.y * r.x + w0.w * r.y + w1.x * r.z;
+    const float zz = w0.z * r.x + w1.x * r.y + w1.y * r.z;
+    d.wx[i] = x;
+    d.wr[i] = r;
+    d.wz[i] = make_float4(zx, zy, zz, 0.f);
+    v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
+    v[1] = (double)r.x * (double)zx + (double)r.y * (double)zy + (double)r.z * (double)zz;
+  }
+  double tot[2];
+  if (scene_reduce<2>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
+    const int it = st->iter_cg + 1;
+    st->iter_cg = it;
+    st->cg_rr = tot[0];
+    const bool conv = tot[0] <= sc.cg_tol2 * st->cg_bb;
+    if (conv || it >= sc.max_iter_cg) {
+      st->active_cg = 0;
+      if (!conv) atomicOr(&st->status, ST_NOCONV);
+      atomicSub(&d.n_active[2], 1);
+    } else {
+      st->cg_beta = tot[1] / st->cg_rz;
+      st->cg_rz = tot[1];
+    }
+  }
+}
+
+static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.P + block - 1) / block); }
+
+void launch_strain(const Dev& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_strain, grid_of(d, block), dim3(block), 0, st, d);
+}
+void launch_cg_init(const Dev& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_init, grid_of(d, block), dim3(block), 0, st, d);
+}
+void launch_cg_matvec(const Dev& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_matvec, grid_of(d, block), dim3(block), 0, st, d);
+}
+void launch_cg_update(const Dev& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d);
+}
+
+}  // namespace nn

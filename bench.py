@@ -1,0 +1,280 @@
+"""Benchmark of the SPH step on one MI355X.
+
+Workload (default): the mixed batch -- 64 independent scenes, dynamic particle
+counts log-uniform in [2k, 60k] (seed 3), viscosity model per scene from
+{Newtonian, Cross, power law}, eta0 log-uniform in [0.1, 100] solved by the
+Weiler implicit solver, 25% of scenes with an elastic block, dt 5e-4.
+
+  value     particle-steps/s of the batched form: all 64 scenes resident in one
+            pool, one launch sequence per tick (a particle-step = one dynamic
+            particle advanced one step; boundary particles are not counted)
+  plain     the same scenes stepped one at a time, one context each
+  latency   p50/p99 scene-step latency: the tick time for the batched form,
+            the per-scene step time for the plain form
+  roofline  the kernel with the largest share of the timed region: algorithmic
+            bytes (per-kernel table below, from the logged per-scene counts and
+            iteration numbers) over its HIP-event time
+  cpu       the oracle (FP32, the reference's arithmetic, OpenMP over particles)
+            on the host cores, on a bounded sample of the same scenes
+
+    python bench.py [--steps K] [--warmup W] [--workload mixed_batch|dam_break|nonnewton_dam|elastic_drop]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from nonnewton_amd import _lib as L  # noqa: E402
+from nonnewton_amd import scenes as S  # noqa: E402
+
+METRIC = "particle-steps/s batched FP32 vs plain; p50/p99 scene-step latency, mixed load"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# algorithmic bytes per launch, per scene (4-byte words; per-particle inputs
+# counted once, one word per neighbor-list entry walked).  N all particles,
+# Nf fluid, E = sum of neighbor counts, Ef = sum over fluid particles.
+# ---------------------------------------------------------------------------
+def alg_bytes(kernel, N, Nf, Ne, E, Ef):
+    w = {
+        "build_lists": 4 * N + E,               # x in; lists + counts out
+        "density_alpha": 8 * N + 4 * Nf + Ef,   # x,V,v,kind in; rho, alpha, rho-dot out
+        "div_update": 5 * N + 9 * Nf + Ef,      # x,V,kind; rho-dot, alpha; v in/out
+        "div_source": 8 * N + 2 * Nf + Ef,      # x,V,v,kind; rho-dot out
+        "press_update": 5 * N + 9 * Nf + Ef,
+        "press_source": 8 * N + 3 * Nf + Ef,    # + rho in
+        "nonpressure": 14 * N + 2 * Nf + Ef,    # x,V,v,kind, rho_j; acc and v' out
+        "cg_matvec": 5 * N + 15 * Nf + Ef,      # x,V,kind; z, p_old, mu, rho in; p, q out
+        "cg_update": N + 27 * Nf,               # p, q, x, r, D^-1 in; x, r, z out
+        "cg_init": 8 * N + 27 * Nf + Ef,
+        "strain": 8 * N + 3 * Nf + Ef,
+        "advect": 12 * N,
+        "clear_acc": 5 * N,
+        "elastic_rot_stress": 23 * Ne,
+        "elastic_force": 26 * Ne,
+    }.get(kernel)
+    return None if w is None else 4 * w
+
+
+# iteration kernels run once per solver iteration of each active scene
+ITER_KERNEL = {"div_update": "iterations_v", "div_source": "iterations_v", "press_update": "iterations_p",
+               "cg_matvec": "iterations_cg", "cg_update": "iterations_cg"}
+
+
+def scene_algorithmic_bytes(kernel, sc, st):
+    b = alg_bytes(kernel, sc.n, sc.n_fluid, sc.n_elastic, st["num_pairs"], st["num_pairs_fluid"])
+    if b is None:
+        return None
+    if kernel in ITER_KERNEL:
+        return b * st[ITER_KERNEL[kernel]]
+    if kernel == "press_source":  # one launch before the loop plus one per iteration
+        return b * (1 + st["iterations_p"])
+    if kernel == "div_source":
+        return b * st["iterations_v"]
+    return b
+
+
+def load_workload(name):
+    if name == "mixed_batch":
+        return S.mixed_batch(), "mixed_batch: 64 scenes, dynamic particles log-uniform 2k-60k, seed 3, dt 5e-4"
+    sc = S.by_name(name)
+    return [sc], f"{name}: {sc.n} particles ({sc.n_dynamic} dynamic), dt {sc.params.dt}"
+
+
+def run_batched(scs, steps, warmup, profile=True):
+    pool = L.Pool(200 << 30)
+    t0 = time.time()
+    slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in scs]
+    if any(q is None for q in slots):
+        raise RuntimeError("pool budget too small for the workload")
+    log(f"[bench] admitted {len(slots)} scenes in {time.time() - t0:.2f} s")
+    for k in range(warmup):
+        pool.tick()
+    if profile:
+        pool.profile(True)
+    lat = []
+    per_step_stats = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        t = time.perf_counter()
+        pool.tick()
+        lat.append(time.perf_counter() - t)
+        per_step_stats.append([pool.stats(q) for q in slots])
+    elapsed = time.perf_counter() - t_start
+    times = pool.kernel_times() if profile else {}
+    pool.profile(False)
+    pool.close()
+    return elapsed, lat, per_step_stats, times
+
+
+def run_plain(scs, steps, warmup):
+    ctxs = []
+    for s in scs:
+        c = L.Context(s.params, s.n)
+        c.upload(s.x, s.kind, s.v)
+        c.init()
+        for k in range(warmup):
+            c.step()
+        ctxs.append(c)
+    lat = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        for c in ctxs:
+            t = time.perf_counter()
+            c.step()
+            lat.append(time.perf_counter() - t)
+    elapsed = time.perf_counter() - t_start
+    for c in ctxs:
+        c.close()
+    return elapsed, lat
+
+
+def cpu_baseline(scs, budget_s):
+    """The oracle in FP32 without compensated sums (the reference's arithmetic)
+    on the host cores: one untimed step, then timed steps, scene after scene,
+    until about budget_s seconds of timed CPU work."""
+    from oracle import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    O.set_threads(cores, "f32")
+    done_dyn = 0
+    spent = 0.0
+    used = []
+    for s in scs:
+        sim = O.OracleSim.from_scene(s, precision="f32", compensated=False)
+        sim.init()
+        sim.step()
+        t = time.perf_counter()
+        sim.step()
+        dt = time.perf_counter() - t
+        spent += dt
+        done_dyn += s.n_dynamic
+        used.append(s.name)
+        if spent >= budget_s:
+            break
+    return {
+        "value": done_dyn / spent,
+        "unit": "particle-steps/s",
+        "cores": int(O.num_threads("f32")),
+        "kind": "port",
+        "sample": f"oracle FP32 (reference arithmetic, OpenMP over particles): one timed step of each of the first "
+                  f"{len(used)} scenes of the workload after one untimed step; {spent:.1f} s of timed CPU work",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--workload", default="mixed_batch")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the SPH scenes are independent and shard across GPUs without exchange; "
+              "run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    scs, workload = load_workload(args.workload)
+    n_dyn = sum(s.n_dynamic for s in scs)
+    n_all = sum(s.n for s in scs)
+    log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
+
+    elapsed, lat, stats, ktimes = run_batched(scs, args.steps, args.warmup)
+    value = n_dyn * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"[bench] batched: {ms_per_step:.3f} ms/tick, {value:.4g} particle-steps/s")
+
+    plain_value = None
+    plain_lat = []
+    if not args.no_plain:
+        pe, plain_lat = run_plain(scs, args.plain_steps, 1)
+        plain_value = n_dyn * args.plain_steps / pe
+        log(f"[bench] plain: {1e3 * pe / args.plain_steps:.3f} ms per round of {len(scs)} scenes, "
+            f"{plain_value:.4g} particle-steps/s")
+
+    # roofline of the dominant kernel
+    roof = None
+    if ktimes:
+        name, (ms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
+        total_bytes = 0.0
+        ok = True
+        for step_stats in stats:
+            for sc, st in zip(scs, step_stats):
+                b = scene_algorithmic_bytes(name, sc, st)
+                if b is None:
+                    ok = False
+                    break
+                total_bytes += b
+        if ok and launches:
+            achieved = total_bytes / (ms * 1e-3) / 1e9
+            roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "launches": launches, "avg_launch_us": round(1e3 * ms / launches, 2),
+                    "bytes_per_launch": round(total_bytes / launches),
+                    "share_of_timed_region": round(ms / (1e3 * elapsed), 3)}
+        log("[bench] kernel times (ms, launches): " +
+            ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in sorted(ktimes.items(), key=lambda kv: -kv[1][0]) if v[1]))
+
+    it = np.array([[st["iterations_v"], st["iterations_p"], st["iterations_cg"]] for ss in stats for st in ss])
+    cfg = {
+        "workload": workload,
+        "scenes": len(scs),
+        "particles": n_all,
+        "dynamic_particles": n_dyn,
+        "batched_ms_per_tick": round(ms_per_step, 3),
+        "p50_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 50)), 3),
+        "p99_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 99)), 3),
+        "mean_iterations_v_p_cg": [round(float(x), 2) for x in it.mean(axis=0)],
+    }
+    if plain_value is not None:
+        cfg["plain_particle_steps_per_s"] = round(plain_value, 1)
+        cfg["batched_over_plain"] = round(value / plain_value, 3)
+        cfg["p50_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 50)), 3)
+        cfg["p99_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 99)), 3)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scs, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lattice scenes, no external data)",
+        "config": cfg,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

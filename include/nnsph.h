@@ -104,7 +104,8 @@ typedef struct nn_step_stats {
     double avg_density_err_p;
     double cg_rel_residual;
     double max_density;
-    int64_t num_pairs;     /* sum of neighbor counts */
+    int64_t num_pairs;       /* sum of neighbor counts */
+    int64_t num_pairs_fluid; /* sum of neighbor counts of fluid particles */
 } nn_step_stats;
 
 /* field ids for nn_download / nn_upload_field (ParticleData, :701-723) */
@@ -194,6 +195,18 @@ int32_t nn_pool_capacity(void);
  * the autotuner sets it.  0 = library default. */
 int32_t nn_pool_set_block(nn_pool *pool, int32_t block);
 int32_t nn_set_block(nn_ctx *ctx, int32_t block);
+
+/* Per-kernel timing with HIP events recorded around every launch on the stream
+ * the kernel runs on (no counterpart in the reference).  enable != 0 turns it
+ * on and clears the totals; nn_*_kernel_times copies the accumulated
+ * milliseconds and launch counts of kernel ids 0..n-1 (names from
+ * nn_kernel_name, NN_KERNEL_COUNT ids). */
+#define NN_KERNEL_COUNT 20
+const char *nn_kernel_name(int32_t id);
+int32_t nn_profile(nn_ctx *ctx, int32_t enable);
+int32_t nn_kernel_times(nn_ctx *ctx, double *ms, int64_t *launches, int32_t n);
+int32_t nn_pool_profile(nn_pool *pool, int32_t enable);
+int32_t nn_pool_kernel_times(nn_pool *pool, double *ms, int64_t *launches, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,7 @@ class NnStepStats(ctypes.Structure):
         ("cg_rel_residual", ctypes.c_double),
         ("max_density", ctypes.c_double),
         ("num_pairs", ctypes.c_int64),
+        ("num_pairs_fluid", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -142,6 +143,11 @@ def load():
         "nn_pool_info": (i32, [vp, P(i32), P(i32), P(i64)]),
         "nn_pool_capacity": (i32, []),
         "nn_pool_set_block": (i32, [vp, i32]),
+        "nn_kernel_name": (ctypes.c_char_p, [i32]),
+        "nn_profile": (i32, [vp, i32]),
+        "nn_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
+        "nn_pool_profile": (i32, [vp, i32]),
+        "nn_pool_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -149,6 +155,21 @@ def load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+KERNEL_COUNT = 20
+
+
+def kernel_names():
+    lib = load()
+    return [lib.nn_kernel_name(k).decode() for k in range(KERNEL_COUNT)]
+
+
+def _times(fn, h):
+    ms = (ctypes.c_double * KERNEL_COUNT)()
+    cnt = (ctypes.c_int64 * KERNEL_COUNT)()
+    check(fn(h, ms, cnt, KERNEL_COUNT), "kernel_times")
+    return {name: (float(ms[k]), int(cnt[k])) for k, name in enumerate(kernel_names())}
 
 
 def last_error() -> str:
@@ -243,6 +264,13 @@ class Context:
     def set_block(self, block: int):
         check(self.lib.nn_set_block(self.h, int(block)), "nn_set_block")
 
+    def profile(self, enable: bool = True):
+        check(self.lib.nn_profile(self.h, int(enable)), "nn_profile")
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} since profiling was enabled."""
+        return _times(self.lib.nn_kernel_times, self.h)
+
 
 class Pool:
     """Many resident scenes stepped together (nn_pool)."""
@@ -313,3 +341,9 @@ class Pool:
 
     def set_block(self, block: int):
         check(self.lib.nn_pool_set_block(self.h, int(block)), "nn_pool_set_block")
+
+    def profile(self, enable: bool = True):
+        check(self.lib.nn_pool_profile(self.h, int(enable)), "nn_pool_profile")
+
+    def kernel_times(self):
+        return _times(self.lib.nn_pool_kernel_times, self.h)

@@ -40,6 +40,29 @@ float neighbor_threshold(float h);
 // device bytes of one scene of n particles, ns stress-set members, c cells
 long long scene_footprint(long long n, long long ns, long long cells);
 
+// Per-kernel durations from HIP events recorded around each launch on the
+// batch stream (enabled on request; read after the step's final sync).
+enum KernelId {
+  K_CELL_COUNT, K_SCAN, K_BUCKET, K_PLACE, K_BUILD_LISTS, K_BVOL, K_CLEAR_ACC, K_DENSITY_ALPHA,
+  K_DIV_UPDATE, K_DIV_SOURCE, K_STRAIN, K_CG_INIT, K_CG_MATVEC, K_CG_UPDATE, K_NONPRESSURE,
+  K_PRESS_SOURCE, K_PRESS_UPDATE, K_ELASTIC_ROT_STRESS, K_ELASTIC_FORCE, K_ADVECT, K_COUNT
+};
+const char* kernel_name(int id);
+
+struct KernelTimer {
+  bool on = false;
+  double total_ms[K_COUNT] = {};
+  long long count[K_COUNT] = {};
+  std::vector<hipEvent_t> events;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> pending;  // kernel id, index of its start event
+  ~KernelTimer();
+  int begin(int id, hipStream_t st);
+  int end(hipStream_t st);
+  int collect();  // after a stream sync
+  void reset();
+};
+
 class Batch {
  public:
   explicit Batch(hipStream_t stream);
@@ -67,6 +90,7 @@ class Batch {
   int block = 256;
   hipStream_t stream;
   Dev d{};
+  KernelTimer timer;
 
  private:
   struct Alloc {

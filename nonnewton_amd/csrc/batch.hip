@@ -215,14 +215,14 @@ void Batch::swap_sort_buffers() {
 int Batch::layout(const std::vector<const Batch*>& from) {
   struct Src {
     const Batch* b;
-    int pb, sb;
+    int pb, sb, ns;  // offsets and stress-set size in the source batch
   };
-  std::vector<Src> src(scenes.size(), Src{nullptr, 0, 0});
+  std::vector<Src> src(scenes.size(), Src{nullptr, 0, 0, 0});
   for (size_t k = 0; k < scenes.size(); k++) {
     for (const Batch* b : from) {
       if (!b) continue;
       for (auto& t : b->scenes)
-        if (t.get() == scenes[k].get()) src[k] = Src{b, scenes[k]->p_begin, scenes[k]->s_begin};
+        if (t.get() == scenes[k].get()) src[k] = Src{b, scenes[k]->p_begin, scenes[k]->s_begin, scenes[k]->dev.ns};
     }
   }
   // offsets
@@ -386,7 +386,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
     HIPCHK(cp(nd.acc + nb, od.acc + ob, sizeof(float4) * n));
     HIPCHK(cp(nd.gam + nb, od.gam + ob, sizeof(float) * n));
     HIPCHK(cp(nd.eta + nb, od.eta + ob, sizeof(float) * n));
-    if (s.ns > 0) {
+    if (s.ns > 0 && src[k].ns == s.ns) {  // a resized stress set is uploaded afresh
       const int os = src[k].sb, ns_ = s.s_begin;
       HIPCHK(cp(nd.s_pid + ns_, od.s_pid + os, sizeof(int) * s.ns));
       HIPCHK(cp(nd.rest_cnt + ns_, od.rest_cnt + os, sizeof(int) * s.ns));
@@ -440,14 +440,73 @@ int Batch::upload(int k, const float* x, const float* v, const int* kind) {
   return NN_OK;
 }
 
+// ---------------------------------------------------------------------------
+// kernel timer
+// ---------------------------------------------------------------------------
+static const char* const KERNEL_NAMES[K_COUNT] = {
+    "cell_count", "scan", "bucket", "place", "build_lists", "bvol", "clear_acc", "density_alpha",
+    "div_update", "div_source", "strain", "cg_init", "cg_matvec", "cg_update", "nonpressure",
+    "press_source", "press_update", "elastic_rot_stress", "elastic_force", "advect"};
+
+const char* kernel_name(int id) { return (id >= 0 && id < K_COUNT) ? KERNEL_NAMES[id] : ""; }
+
+KernelTimer::~KernelTimer() {
+  for (auto e : events) (void)hipEventDestroy(e);
+}
+
+int KernelTimer::begin(int id, hipStream_t st) {
+  if (!on) return NN_OK;
+  while (events.size() < used + 2) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    events.push_back(e);
+  }
+  pending.push_back({id, used});
+  HIPCHK(hipEventRecord(events[used], st));
+  used += 2;
+  return NN_OK;
+}
+
+int KernelTimer::end(hipStream_t st) {
+  if (!on) return NN_OK;
+  HIPCHK(hipEventRecord(events[pending.back().second + 1], st));
+  return NN_OK;
+}
+
+int KernelTimer::collect() {
+  for (auto& p : pending) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, events[p.second], events[p.second + 1]));
+    total_ms[p.first] += ms;
+    count[p.first]++;
+  }
+  pending.clear();
+  used = 0;
+  return NN_OK;
+}
+
+void KernelTimer::reset() {
+  for (int k = 0; k < K_COUNT; k++) {
+    total_ms[k] = 0.0;
+    count[k] = 0;
+  }
+}
+
+#define TIMED(id, call)               \
+  do {                                \
+    CHK(timer.begin((id), stream));   \
+    call;                             \
+    CHK(timer.end(stream));           \
+  } while (0)
+
 int Batch::search_impl() {
   launch_fill_i32(d.cell_count, (long)C_total_ + 1, 0, stream);
-  launch_cell_count(d, block, stream);
-  launch_scan(d, C_total_ + 1, stream);
-  launch_bucket(d, block, stream);
-  launch_place(d, block, stream);
+  TIMED(K_CELL_COUNT, launch_cell_count(d, block, stream));
+  TIMED(K_SCAN, launch_scan(d, C_total_ + 1, stream));
+  TIMED(K_BUCKET, launch_bucket(d, block, stream));
+  TIMED(K_PLACE, launch_place(d, block, stream));
   swap_sort_buffers();
-  launch_build_lists(d, block, stream);
+  TIMED(K_BUILD_LISTS, launch_build_lists(d, block, stream));
   return NN_OK;
 }
 
@@ -528,7 +587,6 @@ int Batch::build_stress_set(const std::vector<int>& which) {
       set_error("an elastic particle has more than 50 rest neighbors");
       return NN_E_REST_OVERFLOW;
     }
-    s.dev.ns = s.ns;
     s_of.swap(s.s_of_host);  // uploaded after the relayout below
   }
   if (!any) return NN_OK;
@@ -565,6 +623,9 @@ int Batch::init_scenes(const std::vector<int>& which) {
     if (stt & ST_NEIGHBOR_OVERFLOW) { set_error("more than 60 neighbors"); return NN_E_NEIGHBOR_OVERFLOW; }
   }
   CHK(build_stress_set(which));
+  // the stress-set relayout does not carry neighbor lists (they hold absolute
+  // slots); search again on the unchanged positions before the volume sweeps
+  CHK(search_impl());
   launch_bvol(d, block, 0, stream);
   launch_bvol(d, block, 1, stream);
   HIPCHK(hipGetLastError());
@@ -584,14 +645,18 @@ int Batch::solver_loop(int which) {
   for (;;) {
     for (int c = 0; c < chunk; c++) {
       if (which == 0) {
-        launch_div_update(d, block, stream);
-        launch_div_source(d, block, stream);
+        TIMED(K_DIV_UPDATE, launch_div_update(d, block, stream));
+        TIMED(K_DIV_SOURCE, launch_div_source(d, block, stream));
+        launch_scene_finish(d, 0, stream);
       } else if (which == 1) {
-        launch_press_update(d, block, stream);
-        launch_press_source(d, block, 1, stream);
+        TIMED(K_PRESS_UPDATE, launch_press_update(d, block, stream));
+        TIMED(K_PRESS_SOURCE, launch_press_source(d, block, 1, stream));
+        launch_scene_finish(d, 1, stream);
       } else {
-        launch_cg_matvec(d, block, stream);
-        launch_cg_update(d, block, stream);
+        TIMED(K_CG_MATVEC, launch_cg_matvec(d, block, stream));
+        launch_scene_finish(d, 3, stream);
+        TIMED(K_CG_UPDATE, launch_cg_update(d, block, stream));
+        launch_scene_finish(d, 4, stream);
         std::swap(d.wp, d.wp_alt);
       }
     }
@@ -613,34 +678,36 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
       return NN_E_STATE;
     }
   launch_step_begin(d, stream);
-  CHK(search_impl());                                  // :917
-  if (any_elastic_) launch_bvol(d, block, 1, stream);  // :918
-  launch_clear_acc(d, block, stream);                  // :920
-  if (any_fluid_) {                                    // DFSPHSolver.substep, :1546-1555
-    launch_density_alpha(d, block, stream);
+  CHK(search_impl());                                                  // :917
+  if (any_elastic_) TIMED(K_BVOL, launch_bvol(d, block, 1, stream));  // :918
+  TIMED(K_CLEAR_ACC, launch_clear_acc(d, block, stream));             // :920
+  if (any_fluid_) {  // DFSPHSolver.substep, :1546-1555
+    TIMED(K_DENSITY_ALPHA, launch_density_alpha(d, block, stream));
     if (any_div_) {
       launch_solver_begin(d, 0, stream);
       CHK(solver_loop(0));
     }
-    if (any_strain_) launch_strain(d, block, stream);
+    if (any_strain_) TIMED(K_STRAIN, launch_strain(d, block, stream));
     if (any_weiler_) {
       launch_solver_begin(d, 2, stream);
-      launch_cg_init(d, block, stream);
+      TIMED(K_CG_INIT, launch_cg_init(d, block, stream));
+      launch_scene_finish(d, 2, stream);
       CHK(solver_loop(2));
     }
-    launch_nonpressure(d, block, stream);
+    TIMED(K_NONPRESSURE, launch_nonpressure(d, block, stream));
     std::swap(d.vel, d.vel_alt);
     launch_solver_begin(d, 1, stream);
-    launch_press_source(d, block, 0, stream);
+    TIMED(K_PRESS_SOURCE, launch_press_source(d, block, 0, stream));
     CHK(solver_loop(1));
   }
   if (any_elastic_) {  // Elasticity.substep, :1241-1244
-    launch_elastic_rot_stress(d, block, stream);
-    launch_elastic_force(d, block, stream);
+    TIMED(K_ELASTIC_ROT_STRESS, launch_elastic_rot_stress(d, block, stream));
+    TIMED(K_ELASTIC_FORCE, launch_elastic_force(d, block, stream));
   }
-  launch_advect(d, block, stream);  // :933, :936-937
+  TIMED(K_ADVECT, launch_advect(d, block, stream));  // :933, :936-937
   HIPCHK(hipGetLastError());
   CHK(gather_state());
+  CHK(timer.collect());
   int rc = NN_OK;
   if (stats) stats->resize(scenes.size());
   for (size_t k = 0; k < scenes.size(); k++) {
@@ -659,6 +726,7 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
     std::memcpy(&mx, &bits, 4);
     o.max_density = mx;
     o.num_pairs = (int64_t)st.pairs;
+    o.num_pairs_fluid = (int64_t)st.pairs_fluid;
     s.last = o;
     s.stepped = true;
     s.last_iter[0] = std::max(1, st.iter_v);

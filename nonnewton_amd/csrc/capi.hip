@@ -219,6 +219,28 @@ int32_t nn_counts(nn_ctx* ctx, int64_t* n, int64_t* n_fluid) {
   return NN_OK;
 }
 
+static int copy_times(const nn::KernelTimer& t, double* ms, int64_t* launches, int32_t n) {
+  for (int k = 0; k < n && k < nn::K_COUNT; k++) {
+    if (ms) ms[k] = t.total_ms[k];
+    if (launches) launches[k] = t.count[k];
+  }
+  return NN_OK;
+}
+
+const char* nn_kernel_name(int32_t id) { return nn::kernel_name(id); }
+
+int32_t nn_profile(nn_ctx* ctx, int32_t enable) {
+  if (!ctx) return NN_E_ARG;
+  ctx->batch->timer.on = enable != 0;
+  ctx->batch->timer.reset();
+  return NN_OK;
+}
+
+int32_t nn_kernel_times(nn_ctx* ctx, double* ms, int64_t* launches, int32_t n) {
+  if (!ctx) return NN_E_ARG;
+  return copy_times(ctx->batch->timer, ms, launches, n);
+}
+
 int32_t nn_set_block(nn_ctx* ctx, int32_t block) {
   if (!ctx) return NN_E_ARG;
   if (block == 0) block = 256;
@@ -472,6 +494,20 @@ int32_t nn_pool_info(nn_pool* p, int32_t* resident, int32_t* pending, int64_t* b
   if (pending) *pending = q;
   if (bytes_in_use) *bytes_in_use = p->in_use;
   return NN_OK;
+}
+
+int32_t nn_pool_profile(nn_pool* p, int32_t enable) {
+  if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->main->timer.on = enable != 0;
+  p->main->timer.reset();
+  return NN_OK;
+}
+
+int32_t nn_pool_kernel_times(nn_pool* p, double* ms, int64_t* launches, int32_t n) {
+  if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return copy_times(p->main->timer, ms, launches, n);
 }
 
 int32_t nn_pool_set_block(nn_pool* p, int32_t block) {

@@ -66,6 +66,37 @@ __device__ __forceinline__ float3 kern_grad(float h, float kG, float rx, float r
   return g;
 }
 
+// Walk the neighbor list of particle i in list order, B entries at a time: the
+// B list words are loaded together, then the B neighbors' data (load(j)), and
+// only then body(j, data) runs for each in order.  The loads of a batch are
+// independent, so a wavefront has B gathers in flight instead of one; the
+// order of the per-neighbor terms (and so every sum) is unchanged.  Lanes past
+// the end of their list load their own particle's data and skip the body.
+template <int B, typename Load, typename Body>
+__device__ __forceinline__ void for_neighbors(const Dev& d, int i, int cnt, Load load, Body body) {
+  using T = decltype(load(0));
+  const long P = d.P;
+  for (int k0 = 0; k0 < cnt; k0 += B) {
+    int j[B];
+    T v[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) j[u] = (k0 + u < cnt) ? d.nbr[(long)(k0 + u) * P + i] : i;
+#pragma unroll
+    for (int u = 0; u < B; u++) v[u] = load(j[u]);
+#pragma unroll
+    for (int u = 0; u < B; u++)
+      if (k0 + u < cnt) body(j[u], v[u]);
+  }
+}
+
+struct PosVel {
+  float4 x, v;
+};
+struct PosSrc {
+  float4 x;
+  float src, alpha;
+};
+
 // Kahan (compensated) summation
 struct KSum {
   float s, c;
@@ -93,35 +124,19 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Per-scene FP64 reduction of NR values.  Each wavefront writes its partial
-// sums; the last wavefront of the scene to arrive (counted with an agent-scope
-// acquire-release atomic) adds the partials in wave order and returns true with
-// the totals in tot[].  Deterministic: the totals do not depend on arrival order.
+// Per-scene FP64 reductions, first stage: every wavefront of a participating
+// scene stores its NR partial sums (plain stores, no fences).  The second stage
+// is k_scene_finish, one wavefront per scene in the next launch, which adds the
+// partials in wave order: the totals are the same on every run and do not depend
+// on the order in which wavefronts ran.
 template <int NR>
-__device__ __forceinline__ bool scene_reduce(const Dev& d, const SceneDev& sc, SceneState* st, int wave,
-                                             double (&v)[NR], double (&tot)[NR]) {
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void wave_partials(const Dev& d, int wave, double (&v)[NR]) {
 #pragma unroll
   for (int r = 0; r < NR; r++) v[r] = wave_sum(v[r]);
-  unsigned old = 0;
-  if (lane == 0) {
+  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int r = 0; r < NR; r++) d.partials[(long)r * d.W + wave] = v[r];
-    old = __hip_atomic_fetch_add(&st->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
   }
-  old = __shfl(old, 0, 64);
-  const unsigned nw = (unsigned)(sc.wave_end - sc.wave_begin);
-  if (old != nw - 1) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    double acc = 0.0;
-    for (int k = sc.wave_begin + lane; k < sc.wave_end; k += 64)
-      acc += __hip_atomic_load(&d.partials[(long)r * d.W + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    tot[r] = wave_sum(acc);
-  }
-  if (lane == 0) __hip_atomic_store(&st->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
 
 // ---------------------------------------------------------------------------

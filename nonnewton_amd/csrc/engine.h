@@ -74,7 +74,7 @@ struct SceneState {
   int active_v, active_p, active_cg, pad_a;
   int iter_v, iter_p, iter_cg, status;
   unsigned counter, max_rho_bits;
-  unsigned long long pairs;
+  unsigned long long pairs, pairs_fluid;
   double err_v, err_p;
   double cg_bb, cg_rr, cg_rz, cg_alpha, cg_beta, cg_pq;
 };
@@ -163,5 +163,6 @@ void launch_elastic_rot_stress(const Dev& d, int block, hipStream_t st);
 void launch_elastic_force(const Dev& d, int block, hipStream_t st);
 void launch_advect(const Dev& d, int block, hipStream_t st);
 void launch_step_begin(const Dev& d, hipStream_t st);
+void launch_scene_finish(const Dev& d, int which, hipStream_t st);  // 0 div, 1 press, 2 cg start, 3 cg alpha, 4 cg test
 
 }  // namespace nn

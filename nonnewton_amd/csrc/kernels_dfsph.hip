@@ -24,6 +24,7 @@ __global__ void k_step_begin(Dev d) {
     d.state[s].max_rho_bits = 0u;
     d.state[s].counter = 0u;
     d.state[s].pairs = 0ull;
+    d.state[s].pairs_fluid = 0ull;
   }
 }
 
@@ -50,30 +51,29 @@ __global__ void k_density_alpha(Dev d) {
     const float4 xi = d.pos[i];
     const float4 vi = d.vel[i];
     const int cnt = d.nnbr[i];
-    const int P = d.P;
     const bool comp = (sc.flags & SF_COMPENSATED) != 0;
     KSum den{0.0f, 0.0f};
     float denp = 0.0f;
     float gix = 0.f, giy = 0.f, giz = 0.f, sk = 0.f, rd = 0.f;
-    for (int k = 0; k < cnt; k++) {
-      const int j = d.nbr[(long)k * P + i];
-      const float4 xj = d.pos[j];
-      const float4 vj = d.vel[j];
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float rn = norm3(rx, ry, rz);
-      const float Vj = fabsf(xj.w);
-      const float t = Vj * kern_w(sc.h, sc.kW, rn);
-      if (comp) den.add(t);
-      else denp += t;
-      const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, rn);
-      const float gpx = -Vj * g.x, gpy = -Vj * g.y, gpz = -Vj * g.z;
-      if (xj.w > 0.0f) sk += gpx * gpx + gpy * gpy + gpz * gpz;
-      gix -= gpx;
-      giy -= gpy;
-      giz -= gpz;
-      const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-      rd += Vj * (dvx * g.x + dvy * g.y + dvz * g.z);
-    }
+    for_neighbors<8>(
+        d, i, cnt, [&](int j) { return PosVel{d.pos[j], d.vel[j]}; },
+        [&](int, const PosVel& nj) {
+          const float4 xj = nj.x, vj = nj.v;
+          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+          const float rn = norm3(rx, ry, rz);
+          const float Vj = fabsf(xj.w);
+          const float t = Vj * kern_w(sc.h, sc.kW, rn);
+          if (comp) den.add(t);
+          else denp += t;
+          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, rn);
+          const float gpx = -Vj * g.x, gpy = -Vj * g.y, gpz = -Vj * g.z;
+          if (xj.w > 0.0f) sk += gpx * gpx + gpy * gpy + gpz * gpz;
+          gix -= gpx;
+          giy -= gpy;
+          giz -= gpz;
+          const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+          rd += Vj * (dvx * g.x + dvy * g.y + dvz * g.z);
+        });
     float rho = xi.w * sc.W0;
     rho += comp ? den.s : denp;
     rho *= sc.rho0;
@@ -134,33 +134,33 @@ __global__ void k_div_update(Dev d) {
   const float4 xi = d.pos[i];
   const float ki = d.src[i] * alpha_div(sc, d.alpha[i]);
   const int cnt = d.nnbr[i];
-  const int P = d.P;
   float dvx = 0.f, dvy = 0.f, dvz = 0.f;
-  for (int k = 0; k < cnt; k++) {
-    const int j = d.nbr[(long)k * P + i];
-    const float4 xj = d.pos[j];
-    if (xj.w > 0.0f) {
-      const float kj = d.src[j] * alpha_div(sc, d.alpha[j]);
-      const float ksum = ki + kj;
-      if (fabsf(ksum) > sc.eps) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-        const float f = sc.dt * ksum;
-        const float Vj = xj.w;
-        dvx -= f * (-Vj * g.x);
-        dvy -= f * (-Vj * g.y);
-        dvz -= f * (-Vj * g.z);
-      }
-    } else if (fabsf(ki) > sc.eps) {
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-      const float f = -sc.dt * ki;
-      const float Vj = -xj.w;
-      dvx += f * (-Vj * g.x);
-      dvy += f * (-Vj * g.y);
-      dvz += f * (-Vj * g.z);
-    }
-  }
+  for_neighbors<8>(
+      d, i, cnt, [&](int j) { return PosSrc{d.pos[j], d.src[j], d.alpha[j]}; },
+      [&](int, const PosSrc& nj) {
+        const float4 xj = nj.x;
+        if (xj.w > 0.0f) {
+          const float kj = nj.src * alpha_div(sc, nj.alpha);
+          const float ksum = ki + kj;
+          if (fabsf(ksum) > sc.eps) {
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+            const float f = sc.dt * ksum;
+            const float Vj = xj.w;
+            dvx -= f * (-Vj * g.x);
+            dvy -= f * (-Vj * g.y);
+            dvz -= f * (-Vj * g.z);
+          }
+        } else if (fabsf(ki) > sc.eps) {
+          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+          const float f = -sc.dt * ki;
+          const float Vj = -xj.w;
+          dvx += f * (-Vj * g.x);
+          dvy += f * (-Vj * g.y);
+          dvz += f * (-Vj * g.z);
+        }
+      });
   float4 v = d.vel[i];
   v.x += dvx;
   v.y += dvy;
@@ -172,17 +172,16 @@ __global__ void k_div_update(Dev d) {
 __device__ __forceinline__ float velocity_divergence(const Dev& d, const SceneDev& sc, int i, float4 xi, int& cnt) {
   const float4 vi = d.vel[i];
   cnt = d.nnbr[i];
-  const int P = d.P;
   float acc = 0.f;
-  for (int k = 0; k < cnt; k++) {
-    const int j = d.nbr[(long)k * P + i];
-    const float4 xj = d.pos[j];
-    const float4 vj = d.vel[j];
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-    const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-    acc += fabsf(xj.w) * (dvx * g.x + dvy * g.y + dvz * g.z);
-  }
+  for_neighbors<8>(
+      d, i, cnt, [&](int j) { return PosVel{d.pos[j], d.vel[j]}; },
+      [&](int, const PosVel& nj) {
+        const float4 xj = nj.x, vj = nj.v;
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+        const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+        acc += fabsf(xj.w) * (dvx * g.x + dvy * g.y + dvz * g.z);
+      });
   return acc;
 }
 
@@ -203,18 +202,7 @@ __global__ void k_div_source(Dev d) {
     d.src[i] = rd;
     v[0] = (double)sc.rho0 * (double)rd - 0.0;
   }
-  double tot[1];
-  if (scene_reduce<1>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
-    const double err = tot[0] / sc.nf_d;
-    const int ran = st->iter_v + 1;
-    st->iter_v = ran;
-    st->err_v = err;
-    if (err <= sc.eta_v || ran >= sc.max_iter_v) {
-      st->active_v = 0;
-      if (!(err <= sc.eta_v)) atomicOr(&st->status, ST_NOCONV);
-      atomicSub(&d.n_active[0], 1);
-    }
-  }
+  wave_partials<1>(d, (int)(i >> 6), v);
 }
 
 // non-pressure forces (:1557-1567): surface tension (:1378-1414) and the
@@ -245,39 +233,42 @@ __global__ void k_nonpressure(Dev d) {
     float sx = 0.f, sy = 0.f, sz = 0.f, wvx = 0.f, wvy = 0.f, wvz = 0.f;
     if (st_on || expl || bnu_on) {
       const int cnt = d.nnbr[i];
-      for (int k = 0; k < cnt; k++) {
-        const int j = d.nbr[(long)k * P + i];
-        const float4 xj = d.pos[j];
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        if (xj.w > 0.0f) {
-          if (st_on) {
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float w = r2 > sc.diam2 ? kern_w(sc.h, sc.kW, norm3(rx, ry, rz)) : sc.Wd;
-            sx -= sc.sig_m * rx * w;
-            sy -= sc.sig_m * ry * w;
-            sz -= sc.sig_m * rz * w;
-          }
-          if (expl) {
-            const float4 vj = d.vel[j];
-            const float vxy = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-            const float rn = norm3(rx, ry, rz);
-            const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, rn);
-            const float f = sc.ten_nu * (sc.mass / d.rho[j]) * vxy / (rn * rn + sc.visc_eps);
-            wvx += f * g.x;
-            wvy += f * g.y;
-            wvz += f * g.z;
-          }
-        } else if (bnu_on) {
-          const float4 vj = d.vel[j];
-          const float vxy = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-          const float rn = norm3(rx, ry, rz);
-          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, rn);
-          const float f = sc.ten_bnu * (sc.rho0 * (-xj.w) / rhoi) * vxy / (rn * rn + sc.visc_eps);
-          wvx += f * g.x;
-          wvy += f * g.y;
-          wvz += f * g.z;
-        }
-      }
+      struct NJ {
+        float4 x, v;
+        float rho;
+      };
+      for_neighbors<8>(
+          d, i, cnt, [&](int j) { return NJ{d.pos[j], d.vel[j], d.rho[j]}; },
+          [&](int, const NJ& nj) {
+            const float4 xj = nj.x, vj = nj.v;
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            if (xj.w > 0.0f) {
+              if (st_on) {
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float w = r2 > sc.diam2 ? kern_w(sc.h, sc.kW, norm3(rx, ry, rz)) : sc.Wd;
+                sx -= sc.sig_m * rx * w;
+                sy -= sc.sig_m * ry * w;
+                sz -= sc.sig_m * rz * w;
+              }
+              if (expl) {
+                const float vxy = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
+                const float rn = norm3(rx, ry, rz);
+                const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, rn);
+                const float f = sc.ten_nu * (sc.mass / nj.rho) * vxy / (rn * rn + sc.visc_eps);
+                wvx += f * g.x;
+                wvy += f * g.y;
+                wvz += f * g.z;
+              }
+            } else if (bnu_on) {
+              const float vxy = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
+              const float rn = norm3(rx, ry, rz);
+              const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, rn);
+              const float f = sc.ten_bnu * (sc.rho0 * (-xj.w) / rhoi) * vxy / (rn * rn + sc.visc_eps);
+              wvx += f * g.x;
+              wvy += f * g.y;
+              wvz += f * g.z;
+            }
+          });
     }
     float ax = sc.gx, ay = sc.gy, az = sc.gz;
     if (st_on) {
@@ -349,17 +340,79 @@ __global__ void k_press_source(Dev d, int reduce) {
     d.src[i] = da;
     v[0] = (double)sc.rho0 * (double)da - sc.rho0_d;
   }
-  if (!reduce) return;
-  double tot[1];
-  if (scene_reduce<1>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
+  if (reduce) wave_partials<1>(d, (int)(i >> 6), v);
+}
+
+// Second stage of the per-scene reductions: one wavefront per scene adds the
+// wave partials in order and applies the loop test of the solver.
+//   which 0: divergence solve (:1695-1703), 1: pressure solve (:1778-1784),
+//         2: CG start, 3: CG step length, 4: CG convergence and direction.
+__global__ void k_scene_finish(Dev d, int which) {
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const SceneDev& sc = d.scenes[s];
+  SceneState* st = d.state + s;
+  const bool weiler = (sc.flags & SF_HAS_FLUID) && (sc.flags & SF_VISCOSITY) && (sc.flags & SF_WEILER);
+  bool part;
+  switch (which) {
+    case 0: part = st->active_v != 0; break;
+    case 1: part = st->active_p != 0; break;
+    case 2: part = weiler; break;
+    default: part = weiler && st->active_cg != 0; break;
+  }
+  if (!part) return;
+  const int nr = which == 2 ? 3 : (which == 4 ? 2 : 1);
+  double tot[3] = {0.0, 0.0, 0.0};
+  for (int r = 0; r < nr; r++) {
+    double acc = 0.0;
+    for (int k = sc.wave_begin + lane; k < sc.wave_end; k += 64) acc += d.partials[(long)r * d.W + k];
+    tot[r] = wave_sum(acc);
+  }
+  if (lane != 0) return;
+  if (which == 0) {
+    const double err = tot[0] / sc.nf_d;
+    const int ran = st->iter_v + 1;
+    st->iter_v = ran;
+    st->err_v = err;
+    if (err <= sc.eta_v || ran >= sc.max_iter_v) {
+      st->active_v = 0;
+      if (!(err <= sc.eta_v)) st->status |= ST_NOCONV;
+      atomicSub(&d.n_active[0], 1);
+    }
+  } else if (which == 1) {
     const double err = tot[0] / sc.nf_d;
     const int ran = st->iter_p + 1;
     st->iter_p = ran;
     st->err_p = err;
     if (err <= sc.eta_p || ran >= sc.max_iter_p) {
       st->active_p = 0;
-      if (!(err <= sc.eta_p)) atomicOr(&st->status, ST_NOCONV);
+      if (!(err <= sc.eta_p)) st->status |= ST_NOCONV;
       atomicSub(&d.n_active[1], 1);
+    }
+  } else if (which == 2) {
+    st->cg_bb = tot[0];
+    st->cg_rr = tot[1];
+    st->cg_rz = tot[2];
+    st->iter_cg = 0;
+    st->cg_beta = 0.0;
+    const int act = (tot[0] > 0.0 && tot[1] > sc.cg_tol2 * tot[0]) ? 1 : 0;
+    st->active_cg = act;
+    if (act) atomicAdd(&d.n_active[2], 1);
+  } else if (which == 3) {
+    st->cg_pq = tot[0];
+    st->cg_alpha = st->cg_rz / tot[0];
+  } else {
+    const int it = st->iter_cg + 1;
+    st->iter_cg = it;
+    st->cg_rr = tot[0];
+    const bool conv = tot[0] <= sc.cg_tol2 * st->cg_bb;
+    if (conv || it >= sc.max_iter_cg) {
+      st->active_cg = 0;
+      if (!conv) st->status |= ST_NOCONV;
+      atomicSub(&d.n_active[2], 1);
+    } else {
+      st->cg_beta = tot[1] / st->cg_rz;
+      st->cg_rz = tot[1];
     }
   }
 }
@@ -380,31 +433,32 @@ __global__ void k_press_update(Dev d) {
   if (kind == KIND_FLUID) {
     const float ki = (d.src[i] - 1.0f) * alpha_press(sc, d.alpha[i]);
     float4 v = d.vel[i];
-    for (int k = 0; k < cnt; k++) {
-      const int j = d.nbr[(long)k * P + i];
-      const float4 xj = d.pos[j];
-      if (xj.w > 0.0f) {
-        const float kj = (d.src[j] - 1.0f) * alpha_press(sc, d.alpha[j]);
-        const float ksum = ki + kj;
-        if (fabsf(ksum) > sc.eps) {
-          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-          const float f = sc.dt * ksum;
-          const float Vj = xj.w;
-          v.x -= f * (-Vj * g.x);
-          v.y -= f * (-Vj * g.y);
-          v.z -= f * (-Vj * g.z);
-        }
-      } else if (fabsf(ki) > sc.eps) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-        const float f = -sc.dt * ki;
-        const float Vj = -xj.w;
-        v.x += f * (-Vj * g.x);
-        v.y += f * (-Vj * g.y);
-        v.z += f * (-Vj * g.z);
-      }
-    }
+    for_neighbors<8>(
+        d, i, cnt, [&](int j) { return PosSrc{d.pos[j], d.src[j], d.alpha[j]}; },
+        [&](int, const PosSrc& nj) {
+          const float4 xj = nj.x;
+          if (xj.w > 0.0f) {
+            const float kj = (nj.src - 1.0f) * alpha_press(sc, nj.alpha);
+            const float ksum = ki + kj;
+            if (fabsf(ksum) > sc.eps) {
+              const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+              const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+              const float f = sc.dt * ksum;
+              const float Vj = xj.w;
+              v.x -= f * (-Vj * g.x);
+              v.y -= f * (-Vj * g.y);
+              v.z -= f * (-Vj * g.z);
+            }
+          } else if (fabsf(ki) > sc.eps) {
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+            const float f = -sc.dt * ki;
+            const float Vj = -xj.w;
+            v.x += f * (-Vj * g.x);
+            v.y += f * (-Vj * g.y);
+            v.z += f * (-Vj * g.z);
+          }
+        });
     d.vel[i] = v;
   } else if (kind == KIND_ELASTIC) {
     const float Vi = -xi.w;
@@ -503,6 +557,9 @@ void launch_press_update(const Dev& d, int block, hipStream_t st) {
 }
 void launch_advect(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d);
+}
+void launch_scene_finish(const Dev& d, int which, hipStream_t st) {
+  if (d.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.nscenes), dim3(64), 0, st, d, which);
 }
 
 }  // namespace nn

@@ -202,15 +202,22 @@ __global__ void k_build_lists(Dev d) {
         const int base = sc.cell_begin + (a * sc.gn1 + b) * sc.gn2;
         const int j0 = d.cell_start[base + z0] + sc.shift;
         const int j1 = d.cell_start[base + z1 + 1] + sc.shift;
-        for (int j = j0; j < j1; j++) {
-          if (j == i) continue;
-          const float4 xj = d.pos[j];
-          const float ex = xi.x - xj.x, ey = xi.y - xj.y, ez = xi.z - xj.z;
-          const float ss = ex * ex + ey * ey + ez * ez;
-          // sqrt_rn(ss) < h  <=>  ss < thr2 (thr2: least float whose rounded sqrt >= h)
-          if (ss < thr2) {
-            if (cnt < MAX_NB) d.nbr[(long)cnt * P + i] = j;
-            cnt++;
+        // candidates in order, 8 positions loaded together
+        for (int jb = j0; jb < j1; jb += 8) {
+          float4 xj[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) xj[u] = d.pos[jb + u < j1 ? jb + u : i];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int j = jb + u;
+            if (j >= j1 || j == i) continue;
+            const float ex = xi.x - xj[u].x, ey = xi.y - xj[u].y, ez = xi.z - xj[u].z;
+            const float ss = ex * ex + ey * ey + ez * ez;
+            // sqrt_rn(ss) < h  <=>  ss < thr2 (thr2: least float whose rounded sqrt >= h)
+            if (ss < thr2) {
+              if (cnt < MAX_NB) d.nbr[(long)cnt * P + i] = j;
+              cnt++;
+            }
           }
         }
       }
@@ -220,11 +227,19 @@ __global__ void k_build_lists(Dev d) {
       cnt = MAX_NB;
     }
   }
-  if (i < sc.p_begin + sc.n) d.nnbr[i] = cnt;
+  int cf = 0;
+  if (i < sc.p_begin + sc.n) {
+    d.nnbr[i] = cnt;
+    if (kind_of(d.info[i]) == KIND_FLUID) cf = cnt;
+  }
   int tot = cnt;
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, 64);
+  for (int off = 32; off >= 1; off >>= 1) {
+    tot += __shfl_xor(tot, off, 64);
+    cf += __shfl_xor(cf, off, 64);
+  }
   if ((threadIdx.x & 63) == 0 && tot > 0) atomicAdd(&d.state[s].pairs, (unsigned long long)tot);
+  if ((threadIdx.x & 63) == 0 && cf > 0) atomicAdd(&d.state[s].pairs_fluid, (unsigned long long)cf);
 }
 
 // compute_static/moving_boundary_volume, SPH/main.py:1010-1035:

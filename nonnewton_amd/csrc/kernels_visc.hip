@@ -43,19 +43,19 @@ __global__ void k_strain(Dev d) {
     float S[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) S[q] = 0.0f;
-    for (int k = 0; k < cnt; k++) {
-      const int j = d.nbr[(long)k * P + i];
-      const float4 xj = d.pos[j];
-      const float4 vj = d.vel[j];
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float3 g3 = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-      const float g[3] = {g3.x, g3.y, g3.z};
-      const float vji[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+    for_neighbors<8>(
+        d, i, cnt, [&](int j) { return PosVel{d.pos[j], d.vel[j]}; },
+        [&](int, const PosVel& nj) {
+          const float4 xj = nj.x, vj = nj.v;
+          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+          const float3 g3 = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+          const float g[3] = {g3.x, g3.y, g3.z};
+          const float vji[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
 #pragma unroll
-      for (int a = 0; a < 3; a++)
+          for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) S[3 * a + b] += sc.mass * (vji[a] * g[b] + vji[b] * g[a]);
-    }
+            for (int b = 0; b < 3; b++) S[3 * a + b] += sc.mass * (vji[a] * g[b] + vji[b] * g[a]);
+        });
     const float f = 0.5f / d.rho[i];
     float n2 = 0.0f;
 #pragma unroll
@@ -95,25 +95,29 @@ __global__ void k_cg_init(Dev d) {
     const int P = d.P;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-    for (int k = 0; k < cnt; k++) {
-      const int j = d.nbr[(long)k * P + i];
-      const float4 xj = d.pos[j];
-      if (!(xj.w > 0.0f)) continue;
-      const float4 vj = d.vel[j];
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-      const float c = weiler_coef(sc, mui, d.mu[j], d.rho[j], rx, ry, rz);
-      const float pr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-      ax += c * pr * g.x;
-      ay += c * pr * g.y;
-      az += c * pr * g.z;
-      m0 += c * g.x * rx;
-      m1 += c * g.x * ry;
-      m2 += c * g.x * rz;
-      m3 += c * g.y * ry;
-      m4 += c * g.y * rz;
-      m5 += c * g.z * rz;
-    }
+    struct NJ {
+      float4 x, v;
+      float mu, rho;
+    };
+    for_neighbors<4>(
+        d, i, cnt, [&](int j) { return NJ{d.pos[j], d.vel[j], d.mu[j], d.rho[j]}; },
+        [&](int, const NJ& nj) {
+          const float4 xj = nj.x, vj = nj.v;
+          if (!(xj.w > 0.0f)) return;
+          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+          const float c = weiler_coef(sc, mui, nj.mu, nj.rho, rx, ry, rz);
+          const float pr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
+          ax += c * pr * g.x;
+          ay += c * pr * g.y;
+          az += c * pr * g.z;
+          m0 += c * g.x * rx;
+          m1 += c * g.x * ry;
+          m2 += c * g.x * rz;
+          m3 += c * g.y * ry;
+          m4 += c * g.y * rz;
+          m5 += c * g.z * rz;
+        });
     const float f = sc.dt / d.rho[i];
     const float qx = vi.x - f * ax, qy = vi.y - f * ay, qz = vi.z - f * az;
     const float rrx = vi.x - qx, rry = vi.y - qy, rrz = vi.z - qz;
@@ -144,17 +148,7 @@ __global__ void k_cg_init(Dev d) {
     v[1] = (double)rrx * (double)rrx + (double)rry * (double)rry + (double)rrz * (double)rrz;
     v[2] = (double)rrx * (double)zx + (double)rry * (double)zy + (double)rrz * (double)zz;
   }
-  double tot[3];
-  if (scene_reduce<3>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
-    st->cg_bb = tot[0];
-    st->cg_rr = tot[1];
-    st->cg_rz = tot[2];
-    st->iter_cg = 0;
-    st->cg_beta = 0.0;
-    const int act = (tot[0] > 0.0 && tot[1] > sc.cg_tol2 * tot[0]) ? 1 : 0;
-    st->active_cg = act;
-    if (act) atomicAdd(&d.n_active[2], 1);
-  }
+  wave_partials<3>(d, (int)(i >> 6), v);
 }
 
 __device__ __forceinline__ float4 cg_dir(const Dev& d, int j, int first, float beta) {
@@ -181,30 +175,33 @@ __global__ void k_cg_matvec(Dev d) {
     const int cnt = d.nnbr[i];
     const int P = d.P;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for (int k = 0; k < cnt; k++) {
-      const int j = d.nbr[(long)k * P + i];
-      const float4 xj = d.pos[j];
-      if (!(xj.w > 0.0f)) continue;
-      const float4 pj = cg_dir(d, j, first, beta);
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-      const float c = weiler_coef(sc, mui, d.mu[j], d.rho[j], rx, ry, rz);
-      const float pr = (pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz;
-      ax += c * pr * g.x;
-      ay += c * pr * g.y;
-      az += c * pr * g.z;
-    }
+    struct NJ {
+      float4 x, z, po;
+      float mu, rho;
+    };
+    for_neighbors<4>(
+        d, i, cnt, [&](int j) { return NJ{d.pos[j], d.wz[j], d.wp_alt[j], d.mu[j], d.rho[j]}; },
+        [&](int, const NJ& nj) {
+          const float4 xj = nj.x;
+          if (!(xj.w > 0.0f)) return;
+          const float4 pj = first ? nj.z
+                                  : make_float4(nj.z.x + beta * nj.po.x, nj.z.y + beta * nj.po.y,
+                                                nj.z.z + beta * nj.po.z, 0.f);
+          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
+          const float c = weiler_coef(sc, mui, nj.mu, nj.rho, rx, ry, rz);
+          const float pr = (pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz;
+          ax += c * pr * g.x;
+          ay += c * pr * g.y;
+          az += c * pr * g.z;
+        });
     const float f = sc.dt / d.rho[i];
     const float qx = pi.x - f * ax, qy = pi.y - f * ay, qz = pi.z - f * az;
     d.wp[i] = pi;
     d.wq[i] = make_float4(qx, qy, qz, 0.f);
     v[0] = (double)pi.x * (double)qx + (double)pi.y * (double)qy + (double)pi.z * (double)qz;
   }
-  double tot[1];
-  if (scene_reduce<1>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
-    st->cg_pq = tot[0];
-    st->cg_alpha = st->cg_rz / tot[0];
-  }
+  wave_partials<1>(d, (int)(i >> 6), v);
 }
 
 // x += alpha p; r -= alpha q; z = D^-1 r; reduce (r.r, r.z); convergence test.
@@ -238,21 +235,7 @@ __global__ void k_cg_update(Dev d) {
     v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[1] = (double)r.x * (double)zx + (double)r.y * (double)zy + (double)r.z * (double)zz;
   }
-  double tot[2];
-  if (scene_reduce<2>(d, sc, st, (int)(i >> 6), v, tot) && (threadIdx.x & 63) == 0) {
-    const int it = st->iter_cg + 1;
-    st->iter_cg = it;
-    st->cg_rr = tot[0];
-    const bool conv = tot[0] <= sc.cg_tol2 * st->cg_bb;
-    if (conv || it >= sc.max_iter_cg) {
-      st->active_cg = 0;
-      if (!conv) atomicOr(&st->status, ST_NOCONV);
-      atomicSub(&d.n_active[2], 1);
-    } else {
-      st->cg_beta = tot[1] / st->cg_rz;
-      st->cg_rz = tot[1];
-    }
-  }
+  wave_partials<2>(d, (int)(i >> 6), v);
 }
 
 static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.P + block - 1) / block); }

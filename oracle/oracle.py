@@ -77,6 +77,8 @@ def _lib(precision: str):
     lib.or_get.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double)]
     lib.or_get.restype = ctypes.c_int32
     lib.or_neighbors.argtypes = [ctypes.c_void_p, P(ctypes.c_int32), P(ctypes.c_int32)]
+    lib.or_set.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double)]
+    lib.or_set.restype = ctypes.c_int32
     lib.or_kernel_w.argtypes = [P(OrParams), ctypes.c_double]
     lib.or_kernel_w.restype = ctypes.c_double
     lib.or_kernel_grad.argtypes = [P(OrParams), P(ctypes.c_double), P(ctypes.c_double)]
@@ -147,6 +149,11 @@ class OracleSim:
         if self.lib.or_get(self.h, field, _ptr(out, ctypes.c_double)) != 0:
             raise ValueError(f"bad field {field}")
         return out.reshape(self.n, w) if w > 1 else out
+
+    def set(self, field: int, values) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n * 3)
+        if self.lib.or_set(self.h, field, _ptr(a, ctypes.c_double)) != 0:
+            raise ValueError("only x and v can be set")
 
     def neighbors(self):
         nbr = np.zeros((self.n, MAX_NB), dtype=np.int32)

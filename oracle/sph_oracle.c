@@ -1283,6 +1283,15 @@ int32_t or_get(const or_sim *s, int32_t field, double *out)
     return 0;
 }
 
+/* overwrite positions or velocities (tests pose deformed states after init) */
+int32_t or_set(or_sim *s, int32_t field, const double *in)
+{
+    real *dst = field == OR_F_X ? s->x : field == OR_F_V ? s->v : NULL;
+    if (!dst) return -1;
+    for (int i = 0; i < 3 * s->n; i++) dst[i] = (real)in[i];
+    return 0;
+}
+
 int32_t or_neighbors(const or_sim *s, int32_t *nbr, int32_t *cnt)
 {
     memcpy(nbr, s->nbr, sizeof(int32_t) * (size_t)s->n * MAX_NB);

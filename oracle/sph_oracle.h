@@ -103,6 +103,7 @@ int32_t or_search(or_sim *s);
 int32_t or_step(or_sim *s, or_stats *st);
 int32_t or_get(const or_sim *s, int32_t field, double *out);
 int32_t or_neighbors(const or_sim *s, int32_t *nbr, int32_t *cnt);
+int32_t or_set(or_sim *s, int32_t field, const double *in);
 int32_t or_num_threads(void);
 void or_set_threads(int32_t n);
 

@@ -1,0 +1,68 @@
+"""The batched/serving pool on the GPU: every scene's results are the same bit
+for bit alone and in the mix (including after retire/admit churn), equal the
+oracle, and admission honours the memory budget."""
+import numpy as np
+import pytest
+
+from nonnewton_amd import _lib as L
+from nonnewton_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(sc, steps):
+    c = L.Context(sc.params, sc.n)
+    c.upload(sc.x, sc.kind, sc.v)
+    c.init()
+    st = None
+    for _ in range(steps):
+        st = c.step()
+    out = {f: c.download(f) for f in (L.FIELD_X, L.FIELD_V, L.FIELD_DENSITY)}
+    c.close()
+    return out, st
+
+
+def test_pool_equals_alone_and_churn():
+    batch = scenes.mixed_batch(count=5, lo=2000, hi=6000)
+    pool = L.Pool(8 << 30)
+    slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in batch[:4]]
+    for _ in range(2):
+        pool.tick()
+    pool.retire(slots[1])
+    s4 = pool.admit(batch[4].params, batch[4].x, batch[4].kind, batch[4].v)
+    pool.tick()
+    r, q, b = pool.info()
+    assert (r, q) == (4, 0)
+    for k, slot, steps in ((0, slots[0], 3), (2, slots[2], 3), (3, slots[3], 3), (4, s4, 1)):
+        ref, st = alone(batch[k], steps)
+        for f, a in ref.items():
+            assert np.array_equal(pool.read(slot, f), a), (k, f)
+        ps = pool.stats(slot)
+        assert (ps["iterations_v"], ps["iterations_p"], ps["iterations_cg"]) == \
+            (st["iterations_v"], st["iterations_p"], st["iterations_cg"])
+    pool.close()
+
+
+def test_pool_newtonian_scene_matches_oracle():
+    rng = np.random.Generator(np.random.PCG64(21))
+    sc = scenes.mixed_scene(rng, 2500, 0, 1.5, 1.0, False, "newt", dt=5e-4)
+    pool = L.Pool(4 << 30)
+    q = pool.admit(sc.params, sc.x, sc.kind)
+    orc = O.OracleSim.from_scene(sc)
+    orc.init()
+    for _ in range(3):
+        pool.tick()
+        orc.step()
+    assert np.array_equal(pool.read(q, L.FIELD_X).astype(np.float64), orc.get(O.F_X))
+    assert np.array_equal(pool.read(q, L.FIELD_V).astype(np.float64), orc.get(O.F_V))
+
+
+def test_pool_budget():
+    sc = scenes.tiny_box()
+    pool = L.Pool(1 << 20)  # 1 MiB
+    assert pool.admit(sc.params, sc.x, sc.kind) is None
+    big = L.Pool(1 << 32)
+    fp = big.footprint(sc.params, sc.n)
+    assert 0 < fp < (1 << 32)
+    assert big.admit(sc.params, sc.x, sc.kind) is not None

@@ -1,0 +1,97 @@
+"""Pins the oracle's non-Newtonian models (NonNewton, SPH/main.py:404-459), the
+strain rate on a linear velocity field, and the Weiler 2018 implicit viscosity
+solve (symmetric positive definite operator, CG vs a dense direct solve)."""
+import numpy as np
+import pytest
+
+from nonnewton_amd.params import VISCOSITY_WEILER, SceneParams
+from oracle import oracle as O
+
+NN = dict(power_index=0.7, consistency_index=1.3, viscosity0=50.0, viscosity_inf=0.1, critical_strain_rate=20.0,
+          mu_c=10.0, yield_stress=200.0, max_viscosity=1e6)
+
+
+def model(method, g):
+    n, K, e0, ei, gc = NN["power_index"], NN["consistency_index"], NN["viscosity0"], NN["viscosity_inf"], NN["critical_strain_rate"]
+    if method == 0:
+        return e0
+    if method == 1:
+        return K * g ** (n - 1)
+    if method == 2:
+        return ei + (e0 - ei) / (1 + (K * g) ** n)
+    if method == 3:
+        return (np.sqrt(NN["mu_c"]) + np.sqrt(NN["yield_stress"] / g)) ** 2
+    if method == 4:
+        return ei + (e0 - ei) / (1 + (K * g * g) ** ((1 - n) / 2))
+    if method == 5:
+        return e0 if g < gc else gc * (e0 - ei) / g + ei
+    if method == 6:
+        return e0 if g < gc else (e0 * gc - K * gc ** n) / g + K * g ** (n - 1)
+
+
+@pytest.mark.parametrize("method", range(7))
+@pytest.mark.parametrize("g", [0.01, 0.5, 3.0, 19.0, 25.0, 400.0])
+def test_models_closed_form(method, g):
+    p = SceneParams(non_newton_method=method, **NN)
+    assert O.viscosity_model(p, g, "f64") == pytest.approx(model(method, g), rel=1e-12)
+    assert O.viscosity_model(p, g, "f32") == pytest.approx(model(method, g), rel=2e-6)
+
+
+def test_model_limits():
+    cross = SceneParams(non_newton_method=2, **NN)
+    assert O.viscosity_model(cross, 1e-12, "f64") == pytest.approx(NN["viscosity0"], rel=1e-6)
+    assert O.viscosity_model(cross, 1e20, "f64") == pytest.approx(NN["viscosity_inf"], rel=1e-6)
+    plaw = SceneParams(non_newton_method=1, **{**NN, "power_index": 1.0})
+    assert O.viscosity_model(plaw, 7.0, "f64") == pytest.approx(NN["consistency_index"])
+    # diverging models are clamped to max_viscosity at zero strain rate
+    assert O.viscosity_model(SceneParams(non_newton_method=1, **{**NN, "max_viscosity": 1e3}), 0.0, "f64") == 1e3
+
+
+def lattice(n, spacing=0.02, origin=0.2):
+    g = np.arange(n) * spacing + origin
+    return np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+
+
+def test_strain_rate_of_linear_field():
+    pts = lattice(9)
+    G = np.array([[0.0, 1.0, 0.0], [0.2, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    v = (pts - pts.mean(axis=0)) @ G.T
+    p = SceneParams(domain_end=(1, 1, 1), gravity=(0, 0, 0), use_surface_tension=False, use_non_newtonian=True,
+                    enable_divergence_solver=False, use_viscosity=False, **NN)
+    s = O.OracleSim(p, pts, np.zeros(len(pts), np.int32), v=v, precision="f64")
+    s.init()
+    s.step()
+    c = 4 * 81 + 4 * 9 + 4
+    sym = 0.5 * (G + G.T)
+    assert s.get(O.F_STRAIN_NORM)[c] == pytest.approx(np.linalg.norm(sym), rel=0.03)
+
+
+def weiler_scene(v, eta0=5.0, method=2):
+    pts = lattice(4, origin=0.3) + np.random.default_rng(1).uniform(-2e-4, 2e-4, (64, 3))
+    p = SceneParams(domain_end=(1, 1, 1), gravity=(0, 0, 0), dt=5e-4, use_non_newtonian=True,
+                    viscosity_method=VISCOSITY_WEILER, weiler_tol=1e-6, weiler_max_iter=500,
+                    **{**NN, "viscosity0": eta0, "non_newton_method": method})
+    return O.OracleSim(p, pts, np.zeros(len(pts), np.int32), v=v, precision="f64")
+
+
+def test_weiler_operator_is_spd_and_cg_solves_it():
+    v = np.random.default_rng(2).normal(0, 0.1, (64, 3))
+    s = weiler_scene(v)
+    s.init()
+    A, b, x, st = s.weiler_probe()
+    assert np.allclose(A, A.T, atol=1e-10 * np.abs(A).max())
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
+    exact = np.linalg.solve(A, b)
+    assert st["iterations_cg"] > 0
+    assert np.linalg.norm(A @ x - b) <= 1.01e-6 * np.linalg.norm(b)
+    assert np.linalg.norm(x - exact) <= 1e-4 * np.linalg.norm(exact)
+    assert np.allclose(b, v.reshape(-1))
+
+
+def test_weiler_leaves_uniform_motion_alone():
+    v = np.tile([0.2, 0.0, -0.1], (64, 1))
+    s = weiler_scene(v)
+    s.init()
+    A, b, x, st = s.weiler_probe()
+    assert st["iterations_cg"] == 0
+    assert np.array_equal(x, b)

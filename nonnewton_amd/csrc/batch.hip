@@ -167,7 +167,7 @@ int derive_scene(SceneHost& s, std::string& err) {
 // bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wx,wr,wz,wp,wp_alt,wq,wd0,x0 (13 float4),
 // wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int), rho,alpha,src,mu,gam,eta (6 float),
 // nbr (60 int), partials (3 doubles per 64 slots), wave table
-static constexpr long long BYTES_PER_SLOT = 13 * 16 + 8 + 8 * 4 + 6 * 4 + MAX_NB * 4 + 1;
+static constexpr long long BYTES_PER_SLOT = 13 * 16 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 2;
 static constexpr long long BYTES_PER_MEMBER = 4 + 4 + MAX_REST * 4 + 4 * 16 + 1;
 
 long long scene_footprint(long long n, long long ns, long long cells) {
@@ -306,7 +306,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
   A(nd.cell_count, ncell_entries);
   A(nd.cell_start, ncell_entries);
   A(nd.scan_tmp, scan_blocks_);
-  A(nd.partials, 3 * (size_t)nd.W);
+  A(nd.partials, 12 * (size_t)nd.W);
   A(nd.n_active, 4);
   A(nd.wx, P);
   A(nd.wr, P);
@@ -316,6 +316,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
   A(nd.wq, P);
   A(nd.wd0, P);
   A(nd.wd1, P);
+  A(nd.spair, (size_t)P * MAX_NB);
   A(nd.mu, P);
   A(nd.gam, P);
   A(nd.eta, P);

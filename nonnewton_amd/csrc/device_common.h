@@ -135,8 +135,50 @@ __device__ __forceinline__ void wave_partials(const Dev& d, int wave, double (&v
   for (int r = 0; r < NR; r++) v[r] = wave_sum(v[r]);
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) d.partials[(long)r * d.W + wave] = v[r];
+    for (int r = 0; r < NR; r++) d.partials[(long)r * 4 * d.W + wave] = v[r];
   }
+}
+
+// Four-lane sweeps: lanes 4i..4i+3 share particle slot i, lane q walks list
+// positions k = q, q+4, ...; a wavefront covers 16 particles.  `hw` is the
+// hardware wavefront index used for the partial sums.
+__device__ __forceinline__ bool quad_slot_scene(const Dev& d, int& i, int& q, int& s, int& hw) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  hw = uniform(t >> 6);
+  i = t >> 2;
+  q = t & 3;
+  const int w = hw >> 2;
+  if (w >= d.W) return false;
+  s = uniform(d.wave_scene[w]);
+  return true;
+}
+
+// (S0 + S1) + (S2 + S3) over the four lanes of a particle; all four lanes end
+// with the same bits.
+__device__ __forceinline__ float quad_sum(float v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  return v;
+}
+
+// Weiler operator pair scalar s_ij = c_ij F(|r|) / (|r| h), c_ij =
+// 10 mu_ij (m / rho_j) / (|r|^2 + 0.01 h^2); 0 where the kernel gradient is 0.
+__device__ __forceinline__ float weiler_pair(const SceneDev& sc, float rx, float ry, float rz, float mui, float muj,
+                                             float rhoj) {
+  const float rn = norm3(rx, ry, rz);
+  const float q = rn / sc.h;
+  if (!(rn > 1e-5f && q <= 1.0f)) return 0.0f;
+  float F;
+  if (q <= 0.5f) {
+    F = (sc.kG * q) * (3.0f * q - 2.0f);
+  } else {
+    const float fa = 1.0f - q;
+    F = sc.kG * ((-fa) * fa);
+  }
+  const float den = rn * sc.h;
+  const float muij = 0.5f * (mui + muj);
+  const float c = 10.0f * muij * (sc.mass / rhoj) / ((rx * rx + ry * ry + rz * rz) + sc.visc_eps);
+  return c * F / den;
 }
 
 // ---------------------------------------------------------------------------

@@ -110,7 +110,7 @@ struct Dev {
   int* cell_count;
   int* cell_start;
   int* scan_tmp;
-  double* partials;  // 3 * W
+  double* partials;  // 3 * 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
   int* n_active;     // [0] divergence, [1] pressure, [2] cg
   // Weiler CG
   float4* wx;
@@ -121,6 +121,7 @@ struct Dev {
   float4* wq;
   float4* wd0;  // inverse diagonal block: (d00, d01, d02, d11)
   float2* wd1;  //                          (d12, d22)
+  float* spair; // per-pair scalar s_ij of the Weiler operator, spair[k * P + i]
   float* mu;    // dynamic viscosity per particle
   float* gam;   // strain-rate norm
   float* eta;   // kinematic viscosity from the model

@@ -362,10 +362,11 @@ __global__ void k_scene_finish(Dev d, int which) {
   }
   if (!part) return;
   const int nr = which == 2 ? 3 : (which == 4 ? 2 : 1);
+  const int g = (which == 2 || which == 3) ? 4 : 1;  // four-lane sweeps write 4 partials per 64 particles
   double tot[3] = {0.0, 0.0, 0.0};
   for (int r = 0; r < nr; r++) {
     double acc = 0.0;
-    for (int k = sc.wave_begin + lane; k < sc.wave_end; k += 64) acc += d.partials[(long)r * d.W + k];
+    for (int k = g * sc.wave_begin + lane; k < g * sc.wave_end; k += 64) acc += d.partials[(long)r * 4 * d.W + k];
     tot[r] = wave_sum(acc);
   }
   if (lane != 0) return;

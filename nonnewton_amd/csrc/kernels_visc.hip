@@ -8,13 +8,18 @@
 //   D_i   = (0.5 / rho_i) sum_j m_j (v_ji (x) gradW_ij + gradW_ij (x) v_ji)
 //   eta_i = model(||D_i||_F)                        (kinematic, :404-459)
 //   mu_i  = rho0 * eta_i   (rho0 * viscosity when the model is off)
-//   (A x)_i = x_i - (dt / rho_i) sum_{fluid j} c_ij ((x_i - x_j) . r_ij) gradW_ij
+//   (A x)_i = x_i - (dt / rho_i) sum_{fluid j} (s_ij ((x_i - x_j) . r_ij)) r_ij
+//   s_ij  = c_ij F(|r_ij|) / (|r_ij| h)   (gradW_ij = F r_ij / (|r_ij| h))
 //   c_ij  = 10 mu_ij (m / rho_j) / (|r_ij|^2 + 0.01 h^2),  mu_ij = (mu_i + mu_j) / 2
 // Weiler et al. 2018, "A Physically Consistent Implicit Viscosity Solver for
 // SPH Fluids", solved by preconditioned conjugate gradients: block-Jacobi 3x3
 // preconditioner, FP64 dot products, stop when ||r|| <= tol ||b|| with b = v.
-// CG iteration = two sweeps: matvec (with the search-direction update
-// p = z + beta p_old fused in, neighbors' p formed on the fly) and update.
+// The pair scalars s_ij are formed once per solve (cg_init) and stored next to
+// the neighbor lists, so a CG matvec needs no sqrt or division.  cg_init and
+// cg_matvec run four lanes per particle, each lane over every fourth list
+// position; list sums are (S0 + S1) + (S2 + S3).  A CG iteration is two sweeps:
+// matvec (with the direction update p = z + beta p_old fused in, neighbors' p
+// formed on the fly) and update.
 #include "device_common.h"
 
 namespace nn {
@@ -39,7 +44,6 @@ __global__ void k_strain(Dev d) {
     const float4 xi = d.pos[i];
     const float4 vi = d.vel[i];
     const int cnt = d.nnbr[i];
-    const int P = d.P;
     float S[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) S[q] = 0.0f;
@@ -71,53 +75,59 @@ __global__ void k_strain(Dev d) {
   if (weiler) d.mu[i] = (nn_on ? eta : sc.nu) * sc.rho0;
 }
 
-// c_ij for a fluid pair (i, j) and its gradient; returns false if j is not fluid
-__device__ __forceinline__ float weiler_coef(const SceneDev& sc, float mui, float muj, float rhoj, float rx, float ry,
-                                             float rz) {
-  const float muij = 0.5f * (mui + muj);
-  return 10.0f * muij * (sc.mass / rhoj) / ((rx * rx + ry * ry + rz * rz) + sc.visc_eps);
-}
-
-// r = b - A x0 with x0 = b = v, the inverse 3x3 diagonal block, z = D^-1 r,
-// p_old = 0; reduces (b.b, r.r, r.z) and activates the scene's CG.
+// Pair scalars s_ij (stored), the inverse diagonal block, r = b - A x0 with
+// x0 = b = v, z = D^-1 r; partial sums of (b.b, r.r, r.z).
 __global__ void k_cg_init(Dev d) {
-  int i, s;
-  if (!slot_scene(d, i, s)) return;
+  int i, q, s, hw;
+  if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
   if (!weiler_scene(sc)) return;
-  SceneState* st = d.state + s;
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
-    const float4 xi = d.pos[i];
-    const float4 vi = d.vel[i];
+  const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
+  float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
+  if (valid) {
+    xi = d.pos[i];
+    vi = d.vel[i];
     const float mui = d.mu[i];
     const int cnt = d.nnbr[i];
-    const int P = d.P;
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-    struct NJ {
-      float4 x, v;
-      float mu, rho;
-    };
-    for_neighbors<4>(
-        d, i, cnt, [&](int j) { return NJ{d.pos[j], d.vel[j], d.mu[j], d.rho[j]}; },
-        [&](int, const NJ& nj) {
-          const float4 xj = nj.x, vj = nj.v;
-          if (!(xj.w > 0.0f)) return;
-          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-          const float c = weiler_coef(sc, mui, nj.mu, nj.rho, rx, ry, rz);
-          const float pr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-          ax += c * pr * g.x;
-          ay += c * pr * g.y;
-          az += c * pr * g.z;
-          m0 += c * g.x * rx;
-          m1 += c * g.x * ry;
-          m2 += c * g.x * rz;
-          m3 += c * g.y * ry;
-          m4 += c * g.y * rz;
-          m5 += c * g.z * rz;
-        });
+    const long P = d.P;
+    for (int k = q; k < cnt; k += 4) {
+      const int j = d.nbr[k * P + i];
+      const float4 xj = d.pos[j];
+      const float4 vj = d.vel[j];
+      const float muj = d.mu[j];
+      const float rhoj = d.rho[j];
+      float sij = 0.0f;
+      if (xj.w > 0.0f) {
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        sij = weiler_pair(sc, rx, ry, rz, mui, muj, rhoj);
+        const float pr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
+        const float t = sij * pr;
+        ax += t * rx;
+        ay += t * ry;
+        az += t * rz;
+        m0 += sij * rx * rx;
+        m1 += sij * rx * ry;
+        m2 += sij * rx * rz;
+        m3 += sij * ry * ry;
+        m4 += sij * ry * rz;
+        m5 += sij * rz * rz;
+      }
+      d.spair[k * P + i] = sij;
+    }
+  }
+  ax = quad_sum(ax);
+  ay = quad_sum(ay);
+  az = quad_sum(az);
+  m0 = quad_sum(m0);
+  m1 = quad_sum(m1);
+  m2 = quad_sum(m2);
+  m3 = quad_sum(m3);
+  m4 = quad_sum(m4);
+  m5 = quad_sum(m5);
+  if (valid && q == 0) {
     const float f = sc.dt / d.rho[i];
     const float qx = vi.x - f * ax, qy = vi.y - f * ay, qz = vi.z - f * az;
     const float rrx = vi.x - qx, rry = vi.y - qy, rrz = vi.z - qz;
@@ -148,68 +158,81 @@ __global__ void k_cg_init(Dev d) {
     v[1] = (double)rrx * (double)rrx + (double)rry * (double)rry + (double)rrz * (double)rrz;
     v[2] = (double)rrx * (double)zx + (double)rry * (double)zy + (double)rrz * (double)zz;
   }
-  wave_partials<3>(d, (int)(i >> 6), v);
+  wave_partials<3>(d, hw, v);
 }
 
-__device__ __forceinline__ float4 cg_dir(const Dev& d, int j, int first, float beta) {
-  const float4 z = d.wz[j];
+__device__ __forceinline__ float4 cg_dir(float4 z, float4 po, bool first, float beta) {
   if (first) return z;
-  const float4 po = d.wp_alt[j];
   return make_float4(z.x + beta * po.x, z.y + beta * po.y, z.z + beta * po.z, 0.f);
 }
 
-// p = z + beta p_old (p = z on the first iteration); q = A p; reduce p.q.
+// p = z + beta p_old (p = z on the first iteration); q = A p; partial p.q.
 __global__ void k_cg_matvec(Dev d) {
-  int i, s;
-  if (!slot_scene(d, i, s)) return;
+  int i, q, s, hw;
+  if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
-  SceneState* st = d.state + s;
+  const SceneState* st = d.state + s;
   if (!weiler_scene(sc) || !st->active_cg) return;
-  const int first = st->iter_cg == 0;
+  const bool first = st->iter_cg == 0;
   const float beta = (float)st->cg_beta;
   double v[1] = {0.0};
-  if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
+  const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) {
     const float4 xi = d.pos[i];
-    const float4 pi = cg_dir(d, i, first, beta);
-    const float mui = d.mu[i];
+    pi = cg_dir(d.wz[i], d.wp_alt[i], first, beta);
     const int cnt = d.nnbr[i];
-    const int P = d.P;
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    struct NJ {
-      float4 x, z, po;
-      float mu, rho;
-    };
-    for_neighbors<4>(
-        d, i, cnt, [&](int j) { return NJ{d.pos[j], d.wz[j], d.wp_alt[j], d.mu[j], d.rho[j]}; },
-        [&](int, const NJ& nj) {
-          const float4 xj = nj.x;
-          if (!(xj.w > 0.0f)) return;
-          const float4 pj = first ? nj.z
-                                  : make_float4(nj.z.x + beta * nj.po.x, nj.z.y + beta * nj.po.y,
-                                                nj.z.z + beta * nj.po.z, 0.f);
-          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-          const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));
-          const float c = weiler_coef(sc, mui, nj.mu, nj.rho, rx, ry, rz);
-          const float pr = (pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz;
-          ax += c * pr * g.x;
-          ay += c * pr * g.y;
-          az += c * pr * g.z;
-        });
+    const long P = d.P;
+    int k = q;
+    // two list positions of this lane per batch: list words, pair scalars,
+    // then the neighbors' positions and directions are loaded together
+    for (; k < cnt; k += 8) {
+      const bool two = k + 4 < cnt;
+      const int j0 = d.nbr[k * P + i];
+      const int j1 = two ? d.nbr[(k + 4) * P + i] : i;
+      const float s0 = d.spair[k * P + i];
+      const float s1 = two ? d.spair[(k + 4) * P + i] : 0.0f;
+      const float4 x0 = d.pos[j0], x1 = d.pos[j1];
+      const float4 z0 = d.wz[j0], z1 = d.wz[j1];
+      const float4 o0 = d.wp_alt[j0], o1 = d.wp_alt[j1];
+      if (x0.w > 0.0f) {
+        const float4 pj = cg_dir(z0, o0, first, beta);
+        const float rx = xi.x - x0.x, ry = xi.y - x0.y, rz = xi.z - x0.z;
+        const float t = s0 * ((pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz);
+        ax += t * rx;
+        ay += t * ry;
+        az += t * rz;
+      }
+      if (two && x1.w > 0.0f) {
+        const float4 pj = cg_dir(z1, o1, first, beta);
+        const float rx = xi.x - x1.x, ry = xi.y - x1.y, rz = xi.z - x1.z;
+        const float t = s1 * ((pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz);
+        ax += t * rx;
+        ay += t * ry;
+        az += t * rz;
+      }
+    }
+  }
+  ax = quad_sum(ax);
+  ay = quad_sum(ay);
+  az = quad_sum(az);
+  if (valid && q == 0) {
     const float f = sc.dt / d.rho[i];
     const float qx = pi.x - f * ax, qy = pi.y - f * ay, qz = pi.z - f * az;
     d.wp[i] = pi;
     d.wq[i] = make_float4(qx, qy, qz, 0.f);
     v[0] = (double)pi.x * (double)qx + (double)pi.y * (double)qy + (double)pi.z * (double)qz;
   }
-  wave_partials<1>(d, (int)(i >> 6), v);
+  wave_partials<1>(d, hw, v);
 }
 
-// x += alpha p; r -= alpha q; z = D^-1 r; reduce (r.r, r.z); convergence test.
+// x += alpha p; r -= alpha q; z = D^-1 r; partial (r.r, r.z).
 __global__ void k_cg_update(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
-  SceneState* st = d.state + s;
+  const SceneState* st = d.state + s;
   if (!weiler_scene(sc) || !st->active_cg) return;
   const float al = (float)st->cg_alpha;
   double v[2] = {0.0, 0.0};
@@ -239,15 +262,16 @@ __global__ void k_cg_update(Dev d) {
 }
 
 static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.P + block - 1) / block); }
+static inline dim3 grid4_of(const Dev& d, int block) { return dim3((4 * d.P + block - 1) / block); }
 
 void launch_strain(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_strain, grid_of(d, block), dim3(block), 0, st, d);
 }
 void launch_cg_init(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_init, grid_of(d, block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(block), 0, st, d);
 }
 void launch_cg_matvec(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_matvec, grid_of(d, block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_cg_matvec, grid4_of(d, block), dim3(block), 0, st, d);
 }
 void launch_cg_update(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d);

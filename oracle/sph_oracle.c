@@ -89,7 +89,7 @@ struct or_sim {
     /* non-Newtonian */
     real *gam, *eta;
     /* Weiler CG work vectors (fluid particles, 3 per particle) */
-    real *wx, *wr, *wz, *wp, *wq, *wdinv, *wmu;
+    real *wx, *wr, *wz, *wp, *wq, *wdinv, *wmu, *wpair;
     int initialized;
 };
 
@@ -646,12 +646,36 @@ static void strain_rate(or_sim *s)
  *   (A x)_i = x_i - (dt/rho_i) sum_{fluid j} c_ij ((x_i - x_j) . r_ij) gradW_ij,
  *   c_ij = 10 mu_ij (m_j/rho_j) / (|r_ij|^2 + 0.01 h^2),  mu_ij = (mu_i + mu_j)/2,
  * mu_i = rho0 * (eta_i if the non-Newtonian model is on, else viscosity).
+ * gradW_ij = F(|r|) r_ij / (|r| h) is parallel to r_ij, so each pair enters
+ * through one scalar s_ij = c_ij F / (|r| h), formed once per solve:
+ *   (A x)_i = x_i - (dt/rho_i) sum_j (s_ij ((x_i - x_j) . r_ij)) r_ij,
+ * and the 3x3 diagonal block is I - (dt/rho_i) sum_j s_ij r_ij (x) r_ij.
+ * Sums over a neighbor list are formed as four partial sums over the list
+ * positions k = 0, 1, 2, 3 mod 4, combined as (S0 + S1) + (S2 + S3).
  * A is symmetric positive definite for a uniform particle mass.  RHS b = v.
- * Solved by conjugate gradients with a 3x3 block-Jacobi preconditioner; dot
+ * Solved by conjugate gradients with the block-Jacobi preconditioner; dot
  * products and norms in FP64; stop when ||r|| <= tol ||b||. */
 static inline real mu_of(const or_sim *s, int i)
 {
     return s->p.use_non_newtonian ? s->eta[i] * s->rho0 : s->nu * s->rho0;
+}
+
+static inline real weiler_pair(const or_sim *s, const real r[3], real mui, real muj, real rhoj)
+{
+    real rn = norm3(r);
+    real q = rn / s->h;
+    if (!(rn > (real)1e-5 && q <= (real)1.0)) return (real)0.0;
+    real F;
+    if (q <= (real)0.5) {
+        F = (s->kG * q) * ((real)3.0 * q - (real)2.0);
+    } else {
+        real fa = (real)1.0 - q;
+        F = s->kG * ((-fa) * fa);
+    }
+    real den = rn * s->h;
+    real muij = (real)0.5 * (mui + muj);
+    real c = (real)10.0 * muij * (s->mass / rhoj) / (dot3(r, r) + s->visc_eps);
+    return c * F / den;
 }
 
 static void weiler_apply(or_sim *s, const real *in, real *out)
@@ -659,47 +683,53 @@ static void weiler_apply(or_sim *s, const real *in, real *out)
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->n; i++) {
         if (!is_fluid(s, i)) continue;
-        real acc[3] = {0, 0, 0};
-        real mui = s->wmu[i];
+        real acc[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         FOR_NB(s, i, k, j)
         {
             if (!is_fluid(s, j)) continue;
-            real r[3], g[3], pd[3];
+            real r[3], pd[3];
             diff3(s->x + 3 * i, s->x + 3 * j, r);
-            kern_grad(s, r, g);
             diff3(in + 3 * i, in + 3 * j, pd);
-            real muij = (real)0.5 * (mui + s->wmu[j]);
-            real c = (real)10.0 * muij * (s->mass / s->rho[j]) / (dot3(r, r) + s->visc_eps);
-            real pr = dot3(pd, r);
-            for (int q = 0; q < 3; q++) acc[q] += c * pr * g[q];
+            real t = s->wpair[(long)i * MAX_NB + k] * dot3(pd, r);
+            for (int q = 0; q < 3; q++) acc[k & 3][q] += t * r[q];
         }
         real f = s->dt / s->rho[i];
-        for (int q = 0; q < 3; q++) out[3 * i + q] = in[3 * i + q] - f * acc[q];
+        for (int q = 0; q < 3; q++) {
+            real a = (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+            out[3 * i + q] = in[3 * i + q] - f * a;
+        }
     }
 }
 
+/* per-pair scalars s_ij and the inverse diagonal blocks */
 static void weiler_precond(or_sim *s)
 {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->n; i++) {
         if (!is_fluid(s, i)) continue;
-        real m[6] = {0, 0, 0, 0, 0, 0}; /* 00 01 02 11 12 22 */
+        real m4[4][6];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 6; b++) m4[a][b] = 0; /* 00 01 02 11 12 22 */
         real mui = s->wmu[i];
         FOR_NB(s, i, k, j)
         {
+            real *sp = s->wpair + (long)i * MAX_NB + k;
+            *sp = 0;
             if (!is_fluid(s, j)) continue;
-            real r[3], g[3];
+            real r[3];
             diff3(s->x + 3 * i, s->x + 3 * j, r);
-            kern_grad(s, r, g);
-            real muij = (real)0.5 * (mui + s->wmu[j]);
-            real c = (real)10.0 * muij * (s->mass / s->rho[j]) / (dot3(r, r) + s->visc_eps);
-            m[0] += c * g[0] * r[0];
-            m[1] += c * g[0] * r[1];
-            m[2] += c * g[0] * r[2];
-            m[3] += c * g[1] * r[1];
-            m[4] += c * g[1] * r[2];
-            m[5] += c * g[2] * r[2];
+            real sij = weiler_pair(s, r, mui, s->wmu[j], s->rho[j]);
+            *sp = sij;
+            real *m = m4[k & 3];
+            m[0] += sij * r[0] * r[0];
+            m[1] += sij * r[0] * r[1];
+            m[2] += sij * r[0] * r[2];
+            m[3] += sij * r[1] * r[1];
+            m[4] += sij * r[1] * r[2];
+            m[5] += sij * r[2] * r[2];
         }
+        real m[6];
+        for (int b = 0; b < 6; b++) m[b] = (m4[0][b] + m4[1][b]) + (m4[2][b] + m4[3][b]);
         real f = s->dt / s->rho[i];
         real d00 = (real)1.0 - f * m[0], d01 = -f * m[1], d02 = -f * m[2];
         real d11 = (real)1.0 - f * m[3], d12 = -f * m[4], d22 = (real)1.0 - f * m[5];
@@ -1120,6 +1150,7 @@ or_sim *or_create(const or_params *p, int32_t n, const float *x, const float *v,
     s->wq = (real *)calloc(n3, sizeof(real));
     s->wdinv = (real *)calloc((size_t)6 * n, sizeof(real));
     s->wmu = (real *)calloc(n, sizeof(real));
+    s->wpair = (real *)calloc((size_t)n * MAX_NB, sizeof(real));
     s->kind = (int32_t *)calloc(n, sizeof(int32_t));
     s->s_of = (int32_t *)calloc(n, sizeof(int32_t));
     s->gcount = (int32_t *)calloc(s->ncells, sizeof(int32_t));
@@ -1146,7 +1177,7 @@ void or_destroy(or_sim *s)
 {
     if (!s) return;
     void *ptrs[] = {s->x, s->v, s->a, s->x0, s->vol, s->rho, s->alpha, s->dadv, s->rest_vol, s->gam, s->eta,
-                    s->wx, s->wr, s->wz, s->wp, s->wq, s->wdinv, s->wmu, s->kind, s->s_of, s->gcount, s->gslot,
+                    s->wx, s->wr, s->wz, s->wp, s->wq, s->wdinv, s->wmu, s->wpair, s->kind, s->s_of, s->gcount, s->gslot,
                     s->nbr, s->nnbr, s->s_pid, s->rest, s->rest_cnt, s->R, s->stress, s->Fm};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(s);
@@ -1369,6 +1400,7 @@ int32_t or_weiler_probe(or_sim *s, double *mat, double *b, double *x, int32_t nm
     int c = 0;
     for (int i = 0; i < s->n; i++) fidx[i] = is_fluid(s, i) ? c++ : -1;
     for (int i = 0; i < s->n; i++) s->wmu[i] = is_fluid(s, i) ? mu_of(s, i) : (real)0.0;
+    weiler_precond(s);
     int m = 3 * nf;
     real *e = (real *)calloc((size_t)3 * s->n, sizeof(real));
     real *o = (real *)calloc((size_t)3 * s->n, sizeof(real));

@@ -143,6 +143,22 @@ def run_plain(scs, steps, warmup):
     return elapsed, lat
 
 
+def run_serving(count, rate):
+    """Continuous batching over a Poisson request stream (sizes log-uniform 1k-500k,
+    50-500 steps, seed 4), 200 GB budget."""
+    from nonnewton_amd.serving import ServingLoop
+
+    reqs = S.serving_requests(seed=4, rate=rate, count=count)
+    loop = ServingLoop(200 << 30)
+    rep = loop.run(reqs)
+    loop.close()
+    s = rep.summary()
+    s["rate_per_s"] = rate
+    s["dynamic_particles_requested"] = int(sum(q.scene.n_dynamic for q in reqs))
+    s["steps_requested"] = int(sum(q.steps for q in reqs))
+    return s
+
+
 def cpu_baseline(scs, budget_s):
     """The oracle in FP32 without compensated sums (the reference's arithmetic)
     on the host cores: one untimed step, then timed steps, scene after scene,
@@ -187,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SPH scenes are independent and shard across GPUs without exchange; "
@@ -250,6 +268,10 @@ def main():
         cfg["batched_over_plain"] = round(value / plain_value, 3)
         cfg["p50_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 50)), 3)
         cfg["p99_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 99)), 3)
+
+    if args.serve_requests > 0 and args.workload == "mixed_batch":
+        cfg["serving"] = run_serving(args.serve_requests, args.serve_rate)
+        log(f"[bench] serving: {cfg['serving']}")
 
     cpu = None
     if not args.no_cpu:

@@ -188,6 +188,9 @@ int32_t nn_pool_admit(nn_pool *pool, const nn_params *params, const float *x, co
  * through *stepped. */
 int32_t nn_pool_tick(nn_pool *pool, nn_step_stats *stats, int32_t stats_len, int32_t *stepped);
 int32_t nn_pool_retire(nn_pool *pool, int32_t slot);
+/* Number of ticks the scene in `slot` has been stepped (0 while it is still
+ * being admitted), -1 for a free slot. */
+int64_t nn_pool_steps(nn_pool *pool, int32_t slot);
 int32_t nn_pool_read(nn_pool *pool, int32_t slot, int32_t field, void *host, size_t bytes);
 int32_t nn_pool_info(nn_pool *pool, int32_t *resident, int32_t *pending, int64_t *bytes_in_use);
 int32_t nn_pool_capacity(void);

@@ -139,6 +139,7 @@ def load():
         "nn_pool_admit": (i32, [vp, P(NnParams), P(ctypes.c_float), P(ctypes.c_float), P(i32), i64, P(i32)]),
         "nn_pool_tick": (i32, [vp, P(NnStepStats), i32, P(i32)]),
         "nn_pool_retire": (i32, [vp, i32]),
+        "nn_pool_steps": (i64, [vp, i32]),
         "nn_pool_read": (i32, [vp, i32, i32, vp, sz]),
         "nn_pool_info": (i32, [vp, P(i32), P(i32), P(i64)]),
         "nn_pool_capacity": (i32, []),
@@ -313,10 +314,17 @@ class Pool:
         self.sizes[slot.value] = x.shape[0]
         return int(slot.value)
 
+    # statuses a scene can report from a step; the other scenes of the tick are unaffected
+    SCENE_ERRORS = (NN_E_NAN, NN_E_OUT_OF_GRID, NN_E_CELL_OVERFLOW, NN_E_NEIGHBOR_OVERFLOW)
+    STATUS_ERROR_BITS = 1 | 2 | 4 | 8
+
     def tick(self):
+        """One step of every resident scene: (rc, scenes stepped).  A scene-level
+        error is returned, not raised; find the scene through stats()['status']."""
         stepped = ctypes.c_int32(0)
         rc = self.lib.nn_pool_tick(self.h, self._stats, self.capacity, ctypes.byref(stepped))
-        check(rc, "nn_pool_tick")
+        if rc not in self.SCENE_ERRORS:
+            check(rc, "nn_pool_tick")
         return rc, int(stepped.value)
 
     def stats(self, slot: int) -> dict:
@@ -325,6 +333,10 @@ class Pool:
     def retire(self, slot: int):
         check(self.lib.nn_pool_retire(self.h, int(slot)), "nn_pool_retire")
         self.sizes.pop(slot, None)
+
+    def steps(self, slot: int) -> int:
+        """Ticks the scene in `slot` has been stepped (0 while still admitted)."""
+        return int(self.lib.nn_pool_steps(self.h, int(slot)))
 
     def read(self, slot: int, field: int) -> np.ndarray:
         n = self.sizes[slot]

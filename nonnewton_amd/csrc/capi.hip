@@ -261,6 +261,7 @@ struct PoolSlot {
   std::unique_ptr<Batch> staging;  // set while the scene waits to join the batch
   hipStream_t staging_stream = nullptr;
   long long bytes = 0;
+  long long steps = 0;  // ticks this scene took part in
   bool used = false, retired = false;
 };
 
@@ -353,6 +354,7 @@ int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const
     p->slots[s_id].used = true;
     p->slots[s_id].retired = false;
     p->slots[s_id].bytes = fp;
+    p->slots[s_id].steps = 0;
     p->in_use += fp;
   }
   PoolSlot& ps = p->slots[s_id];
@@ -448,13 +450,18 @@ int32_t nn_pool_tick(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_
   if (p->main->scenes.empty()) return NN_OK;
   std::vector<nn_step_stats> st;
   rc = p->main->step(&st);
-  if (stats) {
-    for (size_t k = 0; k < st.size(); k++) {
-      const int s = p->member_slot[k];
-      if (s < stats_len) stats[s] = st[k];
-    }
+  for (size_t k = 0; k < st.size(); k++) {
+    const int s = p->member_slot[k];
+    p->slots[s].steps++;
+    if (stats && s < stats_len) stats[s] = st[k];
   }
   return rc;
+}
+
+int64_t nn_pool_steps(nn_pool* p, int32_t slot) {
+  if (!p || slot < 0 || slot >= POOL_CAPACITY) return -1;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return p->slots[slot].used ? p->slots[slot].steps : -1;
 }
 
 static Batch* pool_locate(nn_pool* p, int slot, int& index) {

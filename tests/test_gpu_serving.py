@@ -1,0 +1,72 @@
+"""Serving loop on the GPU: requests admitted while others run give the same bits
+as when run alone; a failing scene is evicted without perturbing the others; a
+tight memory budget queues requests until retirements free it."""
+import numpy as np
+import pytest
+
+from nonnewton_amd import _lib as L
+from nonnewton_amd import scenes
+from nonnewton_amd.scenes import Request
+from nonnewton_amd.serving import ServingLoop
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(sc, steps):
+    c = L.Context(sc.params, sc.n)
+    c.upload(sc.x, sc.kind, sc.v)
+    c.init()
+    for _ in range(steps):
+        c.step()
+    out = {"x": c.download(L.FIELD_X), "v": c.download(L.FIELD_V)}
+    c.close()
+    return out
+
+
+def requests(steps=(3, 5, 2, 4), seed=3):
+    batch = scenes.mixed_batch(seed=seed, count=len(steps), lo=2000, hi=4000)
+    return [Request(k, 0.02 * k, sc, st) for k, (sc, st) in enumerate(zip(batch, steps))]
+
+
+def test_serving_results_equal_alone():
+    reqs = requests()
+    loop = ServingLoop(8 << 30, keep_results=True)
+    rep = loop.run(reqs)
+    loop.close()
+    s = rep.summary()
+    assert s["requests"] == 4 and all(r.finished is not None for r in rep.records)
+    for rec, q in zip(rep.records, reqs):
+        ref = alone(q.scene, q.steps)
+        assert rec.done_steps == q.steps
+        assert np.array_equal(rec.result["x"], ref["x"]) and np.array_equal(rec.result["v"], ref["v"])
+    assert s["p99_step_ms"] >= s["p50_step_ms"] > 0
+
+
+def test_serving_evicts_failing_scene():
+    reqs = requests(steps=(3, 3, 3))
+    bad = reqs[1].scene
+    v = bad.v.copy()
+    v[0] = np.nan  # forced from the host: the scene's state turns non-finite
+    bad.v = v
+    loop = ServingLoop(8 << 30, keep_results=True)
+    rep = loop.run(reqs)
+    loop.close()
+    assert rep.records[1].failed and rep.records[1].done_steps == 1
+    for k in (0, 2):
+        ref = alone(reqs[k].scene, 3)
+        assert not rep.records[k].failed
+        assert np.array_equal(rep.records[k].result["x"], ref["x"])
+
+
+def test_serving_budget_queues_requests():
+    reqs = requests(steps=(2, 2, 2, 2), seed=5)
+    pool = L.Pool(1 << 34)
+    fp = max(pool.footprint(q.scene.params, q.scene.n, q.scene.n_elastic) for q in reqs)
+    pool.close()
+    loop = ServingLoop(int(3.5 * fp), keep_results=True)
+    rep = loop.run(reqs)
+    loop.close()
+    assert rep.peak_resident <= 2
+    assert all(r.finished is not None and not r.failed for r in rep.records)
+    for rec, q in zip(rep.records, reqs):
+        assert np.array_equal(rec.result["x"], alone(q.scene, q.steps)["x"])

@@ -201,14 +201,18 @@ int32_t nn_set_block(nn_ctx *ctx, int32_t block);
 
 /* Per-kernel timing with HIP events recorded around every launch on the stream
  * the kernel runs on (no counterpart in the reference).  enable != 0 turns it
- * on and clears the totals; nn_*_kernel_times copies the accumulated
- * milliseconds and launch counts of kernel ids 0..n-1 (names from
- * nn_kernel_name, NN_KERNEL_COUNT ids). */
-#define NN_KERNEL_COUNT 20
+ * on for every kernel and clears the totals; the _mask forms time only the
+ * kernel ids whose bit is set (0 = off), so the other launches run without
+ * events between them.  nn_*_kernel_times copies the accumulated milliseconds
+ * and launch counts of kernel ids 0..n-1 (names from nn_kernel_name,
+ * NN_KERNEL_COUNT ids). */
+#define NN_KERNEL_COUNT 21
 const char *nn_kernel_name(int32_t id);
 int32_t nn_profile(nn_ctx *ctx, int32_t enable);
+int32_t nn_profile_mask(nn_ctx *ctx, uint32_t mask);
 int32_t nn_kernel_times(nn_ctx *ctx, double *ms, int64_t *launches, int32_t n);
 int32_t nn_pool_profile(nn_pool *pool, int32_t enable);
+int32_t nn_pool_profile_mask(nn_pool *pool, uint32_t mask);
 int32_t nn_pool_kernel_times(nn_pool *pool, double *ms, int64_t *launches, int32_t n);
 
 #ifdef __cplusplus

@@ -146,8 +146,10 @@ def load():
         "nn_pool_set_block": (i32, [vp, i32]),
         "nn_kernel_name": (ctypes.c_char_p, [i32]),
         "nn_profile": (i32, [vp, i32]),
+        "nn_profile_mask": (i32, [vp, ctypes.c_uint32]),
         "nn_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
         "nn_pool_profile": (i32, [vp, i32]),
+        "nn_pool_profile_mask": (i32, [vp, ctypes.c_uint32]),
         "nn_pool_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
     }
     for name, (res, args) in sig.items():
@@ -158,7 +160,18 @@ def load():
     return lib
 
 
-KERNEL_COUNT = 20
+KERNEL_COUNT = 21
+
+
+def kernel_mask(kernels) -> int:
+    """Bit mask of the named kernels for the _mask profiling calls."""
+    names = kernel_names()
+    m = 0
+    for k in kernels:
+        if k not in names:
+            raise ValueError(f"unknown kernel {k!r}")
+        m |= 1 << names.index(k)
+    return m
 
 
 def kernel_names():
@@ -265,8 +278,12 @@ class Context:
     def set_block(self, block: int):
         check(self.lib.nn_set_block(self.h, int(block)), "nn_set_block")
 
-    def profile(self, enable: bool = True):
-        check(self.lib.nn_profile(self.h, int(enable)), "nn_profile")
+    def profile(self, enable: bool = True, kernels=None):
+        """Time every kernel (kernels None) or only the named ones."""
+        if kernels is None:
+            check(self.lib.nn_profile(self.h, int(enable)), "nn_profile")
+        else:
+            check(self.lib.nn_profile_mask(self.h, kernel_mask(kernels) if enable else 0), "nn_profile_mask")
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since profiling was enabled."""
@@ -354,8 +371,13 @@ class Pool:
     def set_block(self, block: int):
         check(self.lib.nn_pool_set_block(self.h, int(block)), "nn_pool_set_block")
 
-    def profile(self, enable: bool = True):
-        check(self.lib.nn_pool_profile(self.h, int(enable)), "nn_pool_profile")
+    def profile(self, enable: bool = True, kernels=None):
+        """Time every kernel (kernels None) or only the named ones."""
+        if kernels is None:
+            check(self.lib.nn_pool_profile(self.h, int(enable)), "nn_pool_profile")
+        else:
+            check(self.lib.nn_pool_profile_mask(self.h, kernel_mask(kernels) if enable else 0),
+                  "nn_pool_profile_mask")
 
     def kernel_times(self):
         return _times(self.lib.nn_pool_kernel_times, self.h)

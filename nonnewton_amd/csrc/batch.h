@@ -45,12 +45,14 @@ long long scene_footprint(long long n, long long ns, long long cells);
 enum KernelId {
   K_CELL_COUNT, K_SCAN, K_BUCKET, K_PLACE, K_BUILD_LISTS, K_BVOL, K_CLEAR_ACC, K_DENSITY_ALPHA,
   K_DIV_UPDATE, K_DIV_SOURCE, K_STRAIN, K_CG_INIT, K_CG_MATVEC, K_CG_UPDATE, K_NONPRESSURE,
-  K_PRESS_SOURCE, K_PRESS_UPDATE, K_ELASTIC_ROT_STRESS, K_ELASTIC_FORCE, K_ADVECT, K_COUNT
+  K_PRESS_SOURCE, K_PRESS_UPDATE, K_ELASTIC_ROT_STRESS, K_ELASTIC_FORCE, K_ADVECT, K_CG_GUESS, K_COUNT
 };
 const char* kernel_name(int id);
 
 struct KernelTimer {
   bool on = false;
+  unsigned mask = ~0u;  // kernel ids timed while on
+  bool open = false;    // a start event was recorded for the launch in flight
   double total_ms[K_COUNT] = {};
   long long count[K_COUNT] = {};
   std::vector<hipEvent_t> events;
@@ -104,6 +106,7 @@ class Batch {
   SceneState* h_state_ = nullptr;  // pinned
   int* h_active_ = nullptr;        // pinned
   bool any_fluid_ = false, any_elastic_ = false, any_weiler_ = false, any_strain_ = false, any_div_ = false;
+  bool any_warm_cg_ = false;
   int max_iter_[3] = {1, 1, 1};
 
   template <class T>

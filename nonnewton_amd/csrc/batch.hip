@@ -61,8 +61,8 @@ int derive_scene(SceneHost& s, std::string& err) {
     err = "particle_radius, dt and density0 must be positive";
     return NN_E_ARG;
   }
-  if (p.warm_start) {
-    err = "warm_start is not supported";
+  if (p.warm_start & ~1) {
+    err = "warm_start: only bit 0 (Weiler CG initial guess) is supported";
     return NN_E_UNSUPPORTED;
   }
   if (p.non_newton_method < 0 || p.non_newton_method > 6) {
@@ -153,6 +153,7 @@ int derive_scene(SceneHost& s, std::string& err) {
     return NN_E_ARG;
   }
   if (p.enable_divergence_solver) f |= SF_DIVERGENCE;
+  if (p.warm_start & 1) f |= SF_WARM_CG;
   if (p.compensated) f |= SF_COMPENSATED;
   if (s.n_fluid > 0) f |= SF_HAS_FLUID;
   if (s.n_elastic > 0) f |= SF_HAS_ELASTIC;
@@ -164,10 +165,10 @@ int derive_scene(SceneHost& s, std::string& err) {
   return NN_OK;
 }
 
-// bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wx,wr,wz,wp,wp_alt,wq,wd0,x0 (13 float4),
+// bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wx,wr,wu,ww,wp,ws,wd0,x0,vdiff (14 float4),
 // wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int), rho,alpha,src,mu,gam,eta (6 float),
-// nbr (60 int), partials (3 doubles per 64 slots), wave table
-static constexpr long long BYTES_PER_SLOT = 13 * 16 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 2;
+// nbr and spair (60 int + 60 float), partials (16 doubles per 64 slots), wave table
+static constexpr long long BYTES_PER_SLOT = 14 * 16 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 3;
 static constexpr long long BYTES_PER_MEMBER = 4 + 4 + MAX_REST * 4 + 4 * 16 + 1;
 
 long long scene_footprint(long long n, long long ns, long long cells) {
@@ -210,6 +211,7 @@ void Batch::swap_sort_buffers() {
   std::swap(d.pos, d.pos_alt);
   std::swap(d.vel, d.vel_alt);
   std::swap(d.info, d.info_alt);
+  std::swap(d.vdiff, d.ws);  // k_place moved the guesses into ws (free outside the solve)
 }
 
 int Batch::layout(const std::vector<const Batch*>& from) {
@@ -228,7 +230,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
   // offsets
   int P = 0, C = 0, NS = 0;
   long base = 0;
-  any_fluid_ = any_elastic_ = any_weiler_ = any_strain_ = any_div_ = false;
+  any_fluid_ = any_elastic_ = any_weiler_ = any_strain_ = any_div_ = any_warm_cg_ = false;
   max_iter_[0] = max_iter_[1] = max_iter_[2] = 1;
   for (auto& sp : scenes) {
     SceneHost& s = *sp;
@@ -251,6 +253,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
       any_fluid_ = true;
       if (f & SF_DIVERGENCE) any_div_ = true;
       if ((f & SF_VISCOSITY) && (f & SF_WEILER)) any_weiler_ = true;
+      if ((f & SF_VISCOSITY) && (f & SF_WEILER) && (f & SF_WARM_CG)) any_warm_cg_ = true;
       if ((f & SF_NON_NEWTONIAN) || ((f & SF_VISCOSITY) && (f & SF_WEILER))) any_strain_ = true;
     }
     if (f & SF_HAS_ELASTIC) any_elastic_ = true;
@@ -306,17 +309,18 @@ int Batch::layout(const std::vector<const Batch*>& from) {
   A(nd.cell_count, ncell_entries);
   A(nd.cell_start, ncell_entries);
   A(nd.scan_tmp, scan_blocks_);
-  A(nd.partials, 12 * (size_t)nd.W);
+  A(nd.partials, 16 * (size_t)nd.W);
   A(nd.n_active, 4);
   A(nd.wx, P);
   A(nd.wr, P);
-  A(nd.wz, P);
+  A(nd.wu, P);
+  A(nd.ww, P);
   A(nd.wp, P);
-  A(nd.wp_alt, P);
-  A(nd.wq, P);
+  A(nd.ws, P);
   A(nd.wd0, P);
   A(nd.wd1, P);
   A(nd.spair, (size_t)P * MAX_NB);
+  A(nd.vdiff, P);
   A(nd.mu, P);
   A(nd.gam, P);
   A(nd.eta, P);
@@ -360,6 +364,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
   launch_fill_f4(nd.vel, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
   launch_fill_f4(nd.vel_alt, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
   launch_fill_f4(nd.x0, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.vdiff, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
   HIPCHK(hipMemsetAsync(nd.rho, 0, sizeof(float) * std::max(P, 1), stream));
   HIPCHK(hipMemsetAsync(nd.alpha, 0, sizeof(float) * std::max(P, 1), stream));
   HIPCHK(hipMemsetAsync(nd.src, 0, sizeof(float) * std::max(P, 1), stream));
@@ -387,6 +392,7 @@ int Batch::layout(const std::vector<const Batch*>& from) {
     HIPCHK(cp(nd.acc + nb, od.acc + ob, sizeof(float4) * n));
     HIPCHK(cp(nd.gam + nb, od.gam + ob, sizeof(float) * n));
     HIPCHK(cp(nd.eta + nb, od.eta + ob, sizeof(float) * n));
+    HIPCHK(cp(nd.vdiff + nb, od.vdiff + ob, sizeof(float4) * n));
     if (s.ns > 0 && src[k].ns == s.ns) {  // a resized stress set is uploaded afresh
       const int os = src[k].sb, ns_ = s.s_begin;
       HIPCHK(cp(nd.s_pid + ns_, od.s_pid + os, sizeof(int) * s.ns));
@@ -435,6 +441,7 @@ int Batch::upload(int k, const float* x, const float* v, const int* kind) {
   HIPCHK(hipMemcpyAsync(d.vel + pb, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(d.info + pb, hi.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(d.inv + pb, hinv.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+  launch_fill_f4(d.vdiff + pb, n, make_float4(0.f, 0.f, 0.f, 0.f), stream);
   HIPCHK(hipStreamSynchronize(stream));
   s.initialized = false;
   s.stepped = false;
@@ -447,7 +454,7 @@ int Batch::upload(int k, const float* x, const float* v, const int* kind) {
 static const char* const KERNEL_NAMES[K_COUNT] = {
     "cell_count", "scan", "bucket", "place", "build_lists", "bvol", "clear_acc", "density_alpha",
     "div_update", "div_source", "strain", "cg_init", "cg_matvec", "cg_update", "nonpressure",
-    "press_source", "press_update", "elastic_rot_stress", "elastic_force", "advect"};
+    "press_source", "press_update", "elastic_rot_stress", "elastic_force", "advect", "cg_guess"};
 
 const char* kernel_name(int id) { return (id >= 0 && id < K_COUNT) ? KERNEL_NAMES[id] : ""; }
 
@@ -456,7 +463,8 @@ KernelTimer::~KernelTimer() {
 }
 
 int KernelTimer::begin(int id, hipStream_t st) {
-  if (!on) return NN_OK;
+  open = on && ((mask >> id) & 1u);
+  if (!open) return NN_OK;
   while (events.size() < used + 2) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
@@ -469,7 +477,8 @@ int KernelTimer::begin(int id, hipStream_t st) {
 }
 
 int KernelTimer::end(hipStream_t st) {
-  if (!on) return NN_OK;
+  if (!open) return NN_OK;
+  open = false;
   HIPCHK(hipEventRecord(events[pending.back().second + 1], st));
   return NN_OK;
 }
@@ -654,11 +663,9 @@ int Batch::solver_loop(int which) {
         TIMED(K_PRESS_SOURCE, launch_press_source(d, block, 1, stream));
         launch_scene_finish(d, 1, stream);
       } else {
-        TIMED(K_CG_MATVEC, launch_cg_matvec(d, block, stream));
-        launch_scene_finish(d, 3, stream);
         TIMED(K_CG_UPDATE, launch_cg_update(d, block, stream));
-        launch_scene_finish(d, 4, stream);
-        std::swap(d.wp, d.wp_alt);
+        TIMED(K_CG_MATVEC, launch_cg_matvec(d, block, 0, stream));
+        launch_scene_finish(d, 3, stream);
       }
     }
     done += chunk;
@@ -692,6 +699,9 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
     if (any_weiler_) {
       launch_solver_begin(d, 2, stream);
       TIMED(K_CG_INIT, launch_cg_init(d, block, stream));
+      if (any_warm_cg_) launch_scene_finish(d, 4, stream);
+      TIMED(K_CG_GUESS, launch_cg_guess(d, block, stream));
+      TIMED(K_CG_MATVEC, launch_cg_matvec(d, block, 1, stream));
       launch_scene_finish(d, 2, stream);
       CHK(solver_loop(2));
     }

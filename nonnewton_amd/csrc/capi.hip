@@ -229,9 +229,12 @@ static int copy_times(const nn::KernelTimer& t, double* ms, int64_t* launches, i
 
 const char* nn_kernel_name(int32_t id) { return nn::kernel_name(id); }
 
-int32_t nn_profile(nn_ctx* ctx, int32_t enable) {
+int32_t nn_profile(nn_ctx* ctx, int32_t enable) { return nn_profile_mask(ctx, enable ? 0xFFFFFFFFu : 0u); }
+
+int32_t nn_profile_mask(nn_ctx* ctx, uint32_t mask) {
   if (!ctx) return NN_E_ARG;
-  ctx->batch->timer.on = enable != 0;
+  ctx->batch->timer.on = mask != 0;
+  ctx->batch->timer.mask = mask;
   ctx->batch->timer.reset();
   return NN_OK;
 }
@@ -503,10 +506,13 @@ int32_t nn_pool_info(nn_pool* p, int32_t* resident, int32_t* pending, int64_t* b
   return NN_OK;
 }
 
-int32_t nn_pool_profile(nn_pool* p, int32_t enable) {
+int32_t nn_pool_profile(nn_pool* p, int32_t enable) { return nn_pool_profile_mask(p, enable ? 0xFFFFFFFFu : 0u); }
+
+int32_t nn_pool_profile_mask(nn_pool* p, uint32_t mask) {
   if (!p) return NN_E_ARG;
   std::lock_guard<std::mutex> lk(p->mu);
-  p->main->timer.on = enable != 0;
+  p->main->timer.on = mask != 0;
+  p->main->timer.mask = mask;
   p->main->timer.reset();
   return NN_OK;
 }

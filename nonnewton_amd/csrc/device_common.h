@@ -125,17 +125,18 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // Per-scene FP64 reductions, first stage: every wavefront of a participating
-// scene stores its NR partial sums (plain stores, no fences).  The second stage
-// is k_scene_finish, one wavefront per scene in the next launch, which adds the
-// partials in wave order: the totals are the same on every run and do not depend
-// on the order in which wavefronts ran.
+// scene stores its NR partial sums into partial arrays base..base+NR-1 (plain
+// stores, no fences).  The second stage is k_scene_finish, one workgroup per
+// scene in the next launch, which adds the partials in a fixed order: the
+// totals are the same on every run and do not depend on the order in which
+// wavefronts ran.
 template <int NR>
-__device__ __forceinline__ void wave_partials(const Dev& d, int wave, double (&v)[NR]) {
+__device__ __forceinline__ void wave_partials(const Dev& d, int wave, double (&v)[NR], int base = 0) {
 #pragma unroll
   for (int r = 0; r < NR; r++) v[r] = wave_sum(v[r]);
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) d.partials[(long)r * 4 * d.W + wave] = v[r];
+    for (int r = 0; r < NR; r++) d.partials[(long)(base + r) * 4 * d.W + wave] = v[r];
   }
 }
 

@@ -44,7 +44,8 @@ enum {
   SF_DIVERGENCE = 16,
   SF_COMPENSATED = 32,
   SF_HAS_FLUID = 64,
-  SF_HAS_ELASTIC = 128
+  SF_HAS_ELASTIC = 128,
+  SF_WARM_CG = 256  // Weiler CG starts from v + (x - v) of the previous solve
 };
 
 // Per-scene constants (device table).  Floats are the reference's FP32
@@ -76,7 +77,7 @@ struct SceneState {
   unsigned counter, max_rho_bits;
   unsigned long long pairs, pairs_fluid;
   double err_v, err_p;
-  double cg_bb, cg_rr, cg_rz, cg_alpha, cg_beta, cg_pq;
+  double cg_bb, cg_rr, cg_gamma, cg_alpha, cg_beta, cg_delta, cg_theta, pad_d;
 };
 
 // Device pointers of a batch (passed by value to every kernel).
@@ -110,18 +111,19 @@ struct Dev {
   int* cell_count;
   int* cell_start;
   int* scan_tmp;
-  double* partials;  // 3 * 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
+  double* partials;  // 4 arrays of 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
   int* n_active;     // [0] divergence, [1] pressure, [2] cg
   // Weiler CG
-  float4* wx;
-  float4* wr;
-  float4* wz;
-  float4* wp;
-  float4* wp_alt;
-  float4* wq;
+  float4* wx;  // iterate x
+  float4* wr;  // residual r
+  float4* wu;  // preconditioned residual u = M r
+  float4* ww;  // w = A u
+  float4* wp;  // direction p
+  float4* ws;  // s = A p
   float4* wd0;  // inverse diagonal block: (d00, d01, d02, d11)
   float2* wd1;  //                          (d12, d22)
   float* spair; // per-pair scalar s_ij of the Weiler operator, spair[k * P + i]
+  float4* vdiff; // x - v of the previous Weiler solve (initial guess; moves with the particle)
   float* mu;    // dynamic viscosity per particle
   float* gam;   // strain-rate norm
   float* eta;   // kinematic viscosity from the model
@@ -153,7 +155,8 @@ void launch_div_source(const Dev& d, int block, hipStream_t st);
 void launch_solver_begin(const Dev& d, int which, hipStream_t st);
 void launch_strain(const Dev& d, int block, hipStream_t st);
 void launch_cg_init(const Dev& d, int block, hipStream_t st);
-void launch_cg_matvec(const Dev& d, int block, hipStream_t st);
+void launch_cg_guess(const Dev& d, int block, hipStream_t st);
+void launch_cg_matvec(const Dev& d, int block, int start, hipStream_t st);
 void launch_cg_update(const Dev& d, int block, hipStream_t st);
 void launch_nonpressure(const Dev& d, int block, hipStream_t st);
 void launch_press_source(const Dev& d, int block, int reduce, hipStream_t st);
@@ -164,6 +167,6 @@ void launch_elastic_rot_stress(const Dev& d, int block, hipStream_t st);
 void launch_elastic_force(const Dev& d, int block, hipStream_t st);
 void launch_advect(const Dev& d, int block, hipStream_t st);
 void launch_step_begin(const Dev& d, hipStream_t st);
-void launch_scene_finish(const Dev& d, int which, hipStream_t st);  // 0 div, 1 press, 2 cg start, 3 cg alpha, 4 cg test
+void launch_scene_finish(const Dev& d, int which, hipStream_t st);  // 0 div, 1 press, 2 cg start, 3 cg iteration, 4 cg guess scale
 
 }  // namespace nn

@@ -282,9 +282,11 @@ __global__ void k_nonpressure(Dev d) {
       az += wvz;
     } else if (weiler) {
       const float4 x = d.wx[i];
-      ax += (x.x - vi.x) / sc.dt;
-      ay += (x.y - vi.y) / sc.dt;
-      az += (x.z - vi.z) / sc.dt;
+      const float dx = x.x - vi.x, dy = x.y - vi.y, dz = x.z - vi.z;
+      ax += dx / sc.dt;
+      ay += dy / sc.dt;
+      az += dz / sc.dt;
+      if (sc.flags & SF_WARM_CG) d.vdiff[i] = make_float4(dx, dy, dz, 0.f);
       if (bnu_on) {
         ax += wvx;
         ay += wvy;
@@ -343,13 +345,20 @@ __global__ void k_press_source(Dev d, int reduce) {
   if (reduce) wave_partials<1>(d, (int)(i >> 6), v);
 }
 
-// Second stage of the per-scene reductions: one wavefront per scene adds the
-// wave partials in order and applies the loop test of the solver.
+// Second stage of the per-scene reductions: one 256-thread workgroup per scene
+// adds the partials of its waves and applies the loop test of the solver.
 //   which 0: divergence solve (:1695-1703), 1: pressure solve (:1778-1784),
-//         2: CG start, 3: CG step length, 4: CG convergence and direction.
-__global__ void k_scene_finish(Dev d, int which) {
+//         2: CG start (after cg_guess and the first matvec),
+//         3: CG iteration (after an update and a matvec),
+//         4: scale of the initial guess (after cg_init, SF_WARM_CG scenes).
+// Thread t adds entries t, t + 256, ... in order; the four wavefront sums are
+// combined as (W0 + W1) + (W2 + W3): a fixed order, the same bits every run.
+constexpr int FINISH_THREADS = 256;
+
+__global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int which) {
+  __shared__ double red[4][FINISH_THREADS / 64];
   const int s = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x;
   const SceneDev& sc = d.scenes[s];
   SceneState* st = d.state + s;
   const bool weiler = (sc.flags & SF_HAS_FLUID) && (sc.flags & SF_VISCOSITY) && (sc.flags & SF_WEILER);
@@ -358,18 +367,33 @@ __global__ void k_scene_finish(Dev d, int which) {
     case 0: part = st->active_v != 0; break;
     case 1: part = st->active_p != 0; break;
     case 2: part = weiler; break;
+    case 4: part = weiler && (sc.flags & SF_WARM_CG); break;
     default: part = weiler && st->active_cg != 0; break;
   }
-  if (!part) return;
-  const int nr = which == 2 ? 3 : (which == 4 ? 2 : 1);
-  const int g = (which == 2 || which == 3) ? 4 : 1;  // four-lane sweeps write 4 partials per 64 particles
-  double tot[3] = {0.0, 0.0, 0.0};
+  if (!part) return;  // uniform over the workgroup
+  // partial arrays read and their granularity (4 = four-lane sweep, a
+  // partial per 16 particles)
+  const int nr = which == 2 ? 4 : (which == 3 ? 3 : (which == 4 ? 2 : 1));
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < nr; r++) {
-    double acc = 0.0;
-    for (int k = g * sc.wave_begin + lane; k < g * sc.wave_end; k += 64) acc += d.partials[(long)r * 4 * d.W + k];
-    tot[r] = wave_sum(acc);
+    // matvec partials (array 3 at the start, 2 in an iteration) and cg_init's
+    // are per 16 particles
+    const int g = which == 4 ? 4 : (which == 2 ? (r == 3 ? 4 : 1) : (which == 3 ? (r == 2 ? 4 : 1) : 1));
+    const double* src = d.partials + (long)r * 4 * d.W;
+    const int lo = g * sc.wave_begin, hi = g * sc.wave_end;
+    double a0 = 0.0, a1 = 0.0;
+    int k = lo + t;
+    for (; k + FINISH_THREADS < hi; k += 2 * FINISH_THREADS) {
+      a0 += src[k];
+      a1 += src[k + FINISH_THREADS];
+    }
+    if (k < hi) a0 += src[k];
+    const double w = wave_sum(a0 + a1);
+    if ((t & 63) == 0) red[r][t >> 6] = w;
   }
-  if (lane != 0) return;
+  __syncthreads();
+  if (t != 0) return;
+  for (int r = 0; r < nr; r++) tot[r] = (red[r][0] + red[r][1]) + (red[r][2] + red[r][3]);
   if (which == 0) {
     const double err = tot[0] / sc.nf_d;
     const int ran = st->iter_v + 1;
@@ -390,19 +414,23 @@ __global__ void k_scene_finish(Dev d, int which) {
       if (!(err <= sc.eta_p)) st->status |= ST_NOCONV;
       atomicSub(&d.n_active[1], 1);
     }
+  } else if (which == 4) {
+    // tot: e.r0, e.Ae
+    st->cg_theta = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
   } else if (which == 2) {
+    // tot: b.b, r.r, r.u, u.w of the initial residual
     st->cg_bb = tot[0];
     st->cg_rr = tot[1];
-    st->cg_rz = tot[2];
-    st->iter_cg = 0;
+    st->cg_gamma = tot[2];
+    st->cg_delta = tot[3];
+    st->cg_alpha = tot[2] / tot[3];
     st->cg_beta = 0.0;
+    st->iter_cg = 0;
     const int act = (tot[0] > 0.0 && tot[1] > sc.cg_tol2 * tot[0]) ? 1 : 0;
     st->active_cg = act;
     if (act) atomicAdd(&d.n_active[2], 1);
-  } else if (which == 3) {
-    st->cg_pq = tot[0];
-    st->cg_alpha = st->cg_rz / tot[0];
   } else {
+    // tot: r.r, r.u (update sweep), u.w (matvec sweep)
     const int it = st->iter_cg + 1;
     st->iter_cg = it;
     st->cg_rr = tot[0];
@@ -412,8 +440,12 @@ __global__ void k_scene_finish(Dev d, int which) {
       if (!conv) st->status |= ST_NOCONV;
       atomicSub(&d.n_active[2], 1);
     } else {
-      st->cg_beta = tot[1] / st->cg_rz;
-      st->cg_rz = tot[1];
+      const double ga = tot[1], de = tot[2];
+      const double be = ga / st->cg_gamma;
+      st->cg_alpha = ga / (de - be * ga / st->cg_alpha);
+      st->cg_beta = be;
+      st->cg_gamma = ga;
+      st->cg_delta = de;
     }
   }
 }
@@ -560,7 +592,7 @@ void launch_advect(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d);
 }
 void launch_scene_finish(const Dev& d, int which, hipStream_t st) {
-  if (d.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.nscenes), dim3(64), 0, st, d, which);
+  if (d.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.nscenes), dim3(FINISH_THREADS), 0, st, d, which);
 }
 
 }  // namespace nn

@@ -135,20 +135,40 @@ __device__ __forceinline__ bool cell_of(const SceneDev& sc, float4 p, int& cx, i
   return cx >= 0 && cx < sc.gn0 && cy >= 0 && cy < sc.gn1 && cz >= 0 && cz < sc.gn2;
 }
 
+// Cell key and a unique rank among the cell's particles.  Particles arrive in
+// the previous step's cell order, so equal keys sit in runs of adjacent lanes:
+// the first lane of each run adds the run length with one atomic and the run's
+// lanes take consecutive ranks from it.  (Ranks only need to be unique; the
+// final order inside a cell is fixed by k_place.)
 __global__ void k_cell_count(Dev d) {
   int i, s;
-  if (!slot_scene(d, i, s)) return;
+  if (!slot_scene(d, i, s)) return;  // uniform over the wavefront
   const SceneDev& sc = d.scenes[s];
-  if (i >= sc.p_begin + sc.n) return;
-  int cx, cy, cz, key;
-  if (cell_of(sc, d.pos[i], cx, cy, cz)) {
-    key = sc.cell_begin + (cx * sc.gn1 + cy) * sc.gn2 + cz;
-  } else {
-    // out of grid: parked in the scene's extra cell, reported, never searched
-    key = sc.cell_begin + sc.gn0 * sc.gn1 * sc.gn2;
-    atomicOr(&d.state[s].status, ST_OUT_OF_GRID);
+  const bool valid = i < sc.p_begin + sc.n;
+  int key = -1;
+  if (valid) {
+    int cx, cy, cz;
+    if (cell_of(sc, d.pos[i], cx, cy, cz)) {
+      key = sc.cell_begin + (cx * sc.gn1 + cy) * sc.gn2 + cz;
+    } else {
+      // out of grid: parked in the scene's extra cell, reported, never searched
+      key = sc.cell_begin + sc.gn0 * sc.gn1 * sc.gn2;
+      atomicOr(&d.state[s].status, ST_OUT_OF_GRID);
+    }
   }
-  int r = atomicAdd(&d.cell_count[key], 1);
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool start = lane == 0 || prev != key;
+  const unsigned long long starts = __ballot(start);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 for lane 63)
+  const int leader = 63 - __clzll(starts & upto);
+  const unsigned long long after = starts & ~upto;
+  const int next = after ? __ffsll(after) - 1 : 64;
+  int base = 0;
+  if (start && key >= 0) base = atomicAdd(&d.cell_count[key], next - lane);
+  base = __shfl(base, leader, 64);
+  if (!valid) return;
+  const int r = base + (lane - leader);
   if (r >= MAX_PIC) atomicOr(&d.state[s].status, ST_CELL_OVERFLOW);
   d.key[i] = key;
   d.rank[i] = r;
@@ -177,6 +197,7 @@ __global__ void k_place(Dev d) {
   d.pos_alt[dest] = d.pos[i];
   d.vel_alt[dest] = d.vel[i];
   d.info_alt[dest] = inf;
+  if (sc.flags & SF_WARM_CG) d.ws[dest] = d.vdiff[i];
   d.inv[sc.p_begin + my] = dest - sc.p_begin;
 }
 

@@ -13,13 +13,19 @@
 //   c_ij  = 10 mu_ij (m / rho_j) / (|r_ij|^2 + 0.01 h^2),  mu_ij = (mu_i + mu_j) / 2
 // Weiler et al. 2018, "A Physically Consistent Implicit Viscosity Solver for
 // SPH Fluids", solved by preconditioned conjugate gradients: block-Jacobi 3x3
-// preconditioner, FP64 dot products, stop when ||r|| <= tol ||b|| with b = v.
+// preconditioner M, FP64 dot products, stop when ||r|| <= tol ||b|| with b = v.
+// A is symmetric (f_i s_ij = 10 dt mu_ij m F / (rho_i rho_j |r| h (|r|^2 +
+// 0.01 h^2)) with mu_ij the mean), so CG applies; it runs in the
+// Chronopoulos-Gear arrangement, which needs one reduction per iteration:
+//   update sweep:  p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
+//                  u = M r; partials (r.r, r.u)
+//   matvec sweep:  w = A u (gathers u of the neighbors); partial u.w
+//   scene finish:  gamma = r.u, delta = u.w, beta = gamma / gamma_old,
+//                  alpha = gamma / (delta - beta gamma / alpha_old), test r.r
 // The pair scalars s_ij are formed once per solve (cg_init) and stored next to
 // the neighbor lists, so a CG matvec needs no sqrt or division.  cg_init and
 // cg_matvec run four lanes per particle, each lane over every fourth list
-// position; list sums are (S0 + S1) + (S2 + S3).  A CG iteration is two sweeps:
-// matvec (with the direction update p = z + beta p_old fused in, neighbors' p
-// formed on the fly) and update.
+// position; list sums are (S0 + S1) + (S2 + S3).
 #include "device_common.h"
 
 namespace nn {
@@ -75,21 +81,27 @@ __global__ void k_strain(Dev d) {
   if (weiler) d.mu[i] = (nn_on ? eta : sc.nu) * sc.rho0;
 }
 
-// Pair scalars s_ij (stored), the inverse diagonal block, r = b - A x0 with
-// x0 = b = v, z = D^-1 r; partial sums of (b.b, r.r, r.z).
+// Start of a solve, b = v.  Pair scalars s_ij (stored), the inverse diagonal
+// block M, r0 = b - A b (into wr).  With SF_WARM_CG also A e for the previous
+// solve's e = x - v (into ww) and partial sums of (e.r0, e.Ae), from which the
+// scene finish forms theta = (e.r0) / (e.Ae); k_cg_guess then starts CG from
+// x0 = b + theta e, r = r0 - theta A e.
 __global__ void k_cg_init(Dev d) {
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
   if (!weiler_scene(sc)) return;
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[2] = {0.0, 0.0};
   const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
-  float ax = 0.f, ay = 0.f, az = 0.f;
+  const bool warm = (sc.flags & SF_WARM_CG) != 0;
+  float ax = 0.f, ay = 0.f, az = 0.f;  // sum for A v
+  float bx = 0.f, by = 0.f, bz = 0.f;  // sum for A e
   float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-  float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
+  float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi, ei = xi;
   if (valid) {
     xi = d.pos[i];
     vi = d.vel[i];
+    if (warm) ei = d.vdiff[i];
     const float mui = d.mu[i];
     const int cnt = d.nnbr[i];
     const long P = d.P;
@@ -103,11 +115,17 @@ __global__ void k_cg_init(Dev d) {
       if (xj.w > 0.0f) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         sij = weiler_pair(sc, rx, ry, rz, mui, muj, rhoj);
-        const float pr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-        const float t = sij * pr;
+        const float t = sij * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
         ax += t * rx;
         ay += t * ry;
         az += t * rz;
+        if (warm) {
+          const float4 ej = d.vdiff[j];
+          const float te = sij * ((ei.x - ej.x) * rx + (ei.y - ej.y) * ry + (ei.z - ej.z) * rz);
+          bx += te * rx;
+          by += te * ry;
+          bz += te * rz;
+        }
         m0 += sij * rx * rx;
         m1 += sij * rx * ry;
         m2 += sij * rx * rz;
@@ -121,6 +139,11 @@ __global__ void k_cg_init(Dev d) {
   ax = quad_sum(ax);
   ay = quad_sum(ay);
   az = quad_sum(az);
+  if (warm) {
+    bx = quad_sum(bx);
+    by = quad_sum(by);
+    bz = quad_sum(bz);
+  }
   m0 = quad_sum(m0);
   m1 = quad_sum(m1);
   m2 = quad_sum(m2);
@@ -146,47 +169,74 @@ __global__ void k_cg_init(Dev d) {
       w0 = make_float4(1.f, 0.f, 0.f, 1.f);
       w1 = make_float2(0.f, 1.f);
     }
-    const float zx = w0.x * rrx + w0.y * rry + w0.z * rrz;
-    const float zy = w0.y * rrx + w0.w * rry + w1.x * rrz;
-    const float zz = w0.z * rrx + w1.x * rry + w1.y * rrz;
     d.wd0[i] = w0;
     d.wd1[i] = w1;
-    d.wx[i] = make_float4(vi.x, vi.y, vi.z, 0.f);
     d.wr[i] = make_float4(rrx, rry, rrz, 0.f);
-    d.wz[i] = make_float4(zx, zy, zz, 0.f);
-    v[0] = (double)vi.x * (double)vi.x + (double)vi.y * (double)vi.y + (double)vi.z * (double)vi.z;
-    v[1] = (double)rrx * (double)rrx + (double)rry * (double)rry + (double)rrz * (double)rrz;
-    v[2] = (double)rrx * (double)zx + (double)rry * (double)zy + (double)rrz * (double)zz;
+    if (warm) {
+      const float zx = ei.x - f * bx, zy = ei.y - f * by, zz = ei.z - f * bz;
+      d.ww[i] = make_float4(zx, zy, zz, 0.f);
+      v[0] = (double)ei.x * (double)rrx + (double)ei.y * (double)rry + (double)ei.z * (double)rrz;
+      v[1] = (double)ei.x * (double)zx + (double)ei.y * (double)zy + (double)ei.z * (double)zz;
+    }
   }
-  wave_partials<3>(d, hw, v);
+  wave_partials<2>(d, hw, v);
 }
 
-__device__ __forceinline__ float4 cg_dir(float4 z, float4 po, bool first, float beta) {
-  if (first) return z;
-  return make_float4(z.x + beta * po.x, z.y + beta * po.y, z.z + beta * po.z, 0.f);
+// x0 = b + theta e, r = r0 - theta A e (x0 = b, r = r0 without SF_WARM_CG),
+// u = M r; partials (b.b, r.r, r.u).
+__global__ void k_cg_guess(Dev d) {
+  int i, s;
+  if (!slot_scene(d, i, s)) return;
+  const SceneDev& sc = d.scenes[s];
+  if (!weiler_scene(sc)) return;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
+    const float4 b = d.vel[i];
+    float4 r = d.wr[i];
+    float4 x = make_float4(b.x, b.y, b.z, 0.f);
+    if (sc.flags & SF_WARM_CG) {
+      const float th = (float)d.state[s].cg_theta;
+      const float4 e = d.vdiff[i];
+      const float4 z = d.ww[i];
+      x = make_float4(b.x + th * e.x, b.y + th * e.y, b.z + th * e.z, 0.f);
+      r = make_float4(r.x - th * z.x, r.y - th * z.y, r.z - th * z.z, 0.f);
+    }
+    const float4 w0 = d.wd0[i];
+    const float2 w1 = d.wd1[i];
+    const float ux = w0.x * r.x + w0.y * r.y + w0.z * r.z;
+    const float uy = w0.y * r.x + w0.w * r.y + w1.x * r.z;
+    const float uz = w0.z * r.x + w1.x * r.y + w1.y * r.z;
+    d.wx[i] = x;
+    d.wr[i] = r;
+    d.wu[i] = make_float4(ux, uy, uz, 0.f);
+    v[0] = (double)b.x * (double)b.x + (double)b.y * (double)b.y + (double)b.z * (double)b.z;
+    v[1] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
+    v[2] = (double)r.x * (double)ux + (double)r.y * (double)uy + (double)r.z * (double)uz;
+  }
+  wave_partials<3>(d, (int)(i >> 6), v);
 }
 
-// p = z + beta p_old (p = z on the first iteration); q = A p; partial p.q.
-__global__ void k_cg_matvec(Dev d) {
+// w = A u (u gathered from the neighbors); partial u.w into partial array
+// 3 (start: after cg_init, every Weiler scene) or 2 (an iteration, active
+// scenes).
+__global__ void k_cg_matvec(Dev d, int start) {
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
   const SceneState* st = d.state + s;
-  if (!weiler_scene(sc) || !st->active_cg) return;
-  const bool first = st->iter_cg == 0;
-  const float beta = (float)st->cg_beta;
+  if (!weiler_scene(sc) || !(start || st->active_cg)) return;
   double v[1] = {0.0};
   const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
   float ax = 0.f, ay = 0.f, az = 0.f;
-  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
   if (valid) {
     const float4 xi = d.pos[i];
-    pi = cg_dir(d.wz[i], d.wp_alt[i], first, beta);
+    ui = d.wu[i];
     const int cnt = d.nnbr[i];
     const long P = d.P;
     int k = q;
     // two list positions of this lane per batch: list words, pair scalars,
-    // then the neighbors' positions and directions are loaded together
+    // then the neighbors' positions and u are loaded together
     for (; k < cnt; k += 8) {
       const bool two = k + 4 < cnt;
       const int j0 = d.nbr[k * P + i];
@@ -194,20 +244,17 @@ __global__ void k_cg_matvec(Dev d) {
       const float s0 = d.spair[k * P + i];
       const float s1 = two ? d.spair[(k + 4) * P + i] : 0.0f;
       const float4 x0 = d.pos[j0], x1 = d.pos[j1];
-      const float4 z0 = d.wz[j0], z1 = d.wz[j1];
-      const float4 o0 = d.wp_alt[j0], o1 = d.wp_alt[j1];
+      const float4 u0 = d.wu[j0], u1 = d.wu[j1];
       if (x0.w > 0.0f) {
-        const float4 pj = cg_dir(z0, o0, first, beta);
         const float rx = xi.x - x0.x, ry = xi.y - x0.y, rz = xi.z - x0.z;
-        const float t = s0 * ((pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz);
+        const float t = s0 * ((ui.x - u0.x) * rx + (ui.y - u0.y) * ry + (ui.z - u0.z) * rz);
         ax += t * rx;
         ay += t * ry;
         az += t * rz;
       }
       if (two && x1.w > 0.0f) {
-        const float4 pj = cg_dir(z1, o1, first, beta);
         const float rx = xi.x - x1.x, ry = xi.y - x1.y, rz = xi.z - x1.z;
-        const float t = s1 * ((pi.x - pj.x) * rx + (pi.y - pj.y) * ry + (pi.z - pj.z) * rz);
+        const float t = s1 * ((ui.x - u1.x) * rx + (ui.y - u1.y) * ry + (ui.z - u1.z) * rz);
         ax += t * rx;
         ay += t * ry;
         az += t * rz;
@@ -219,42 +266,56 @@ __global__ void k_cg_matvec(Dev d) {
   az = quad_sum(az);
   if (valid && q == 0) {
     const float f = sc.dt / d.rho[i];
-    const float qx = pi.x - f * ax, qy = pi.y - f * ay, qz = pi.z - f * az;
-    d.wp[i] = pi;
-    d.wq[i] = make_float4(qx, qy, qz, 0.f);
-    v[0] = (double)pi.x * (double)qx + (double)pi.y * (double)qy + (double)pi.z * (double)qz;
+    const float wx = ui.x - f * ax, wy = ui.y - f * ay, wz = ui.z - f * az;
+    d.ww[i] = make_float4(wx, wy, wz, 0.f);
+    v[0] = (double)ui.x * (double)wx + (double)ui.y * (double)wy + (double)ui.z * (double)wz;
   }
-  wave_partials<1>(d, hw, v);
+  wave_partials<1>(d, hw, v, start ? 3 : 2);
 }
 
-// x += alpha p; r -= alpha q; z = D^-1 r; partial (r.r, r.z).
+// p = u + beta p, s = w + beta s (p = u, s = w on the first iteration);
+// x += alpha p; r -= alpha s; u = M r; partials (r.r, r.u).
 __global__ void k_cg_update(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
   const SceneState* st = d.state + s;
   if (!weiler_scene(sc) || !st->active_cg) return;
+  const bool first = st->iter_cg == 0;
   const float al = (float)st->cg_alpha;
+  const float be = (float)st->cg_beta;
   double v[2] = {0.0, 0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
-    const float4 p = d.wp[i];
-    const float4 q = d.wq[i];
+    const float4 u = d.wu[i];
+    const float4 w = d.ww[i];
+    float4 p, sv;
+    if (first) {
+      p = u;
+      sv = w;
+    } else {
+      const float4 po = d.wp[i];
+      const float4 so = d.ws[i];
+      p = make_float4(u.x + be * po.x, u.y + be * po.y, u.z + be * po.z, 0.f);
+      sv = make_float4(w.x + be * so.x, w.y + be * so.y, w.z + be * so.z, 0.f);
+    }
     float4 x = d.wx[i];
     float4 r = d.wr[i];
     x.x += al * p.x;
     x.y += al * p.y;
     x.z += al * p.z;
-    r.x -= al * q.x;
-    r.y -= al * q.y;
-    r.z -= al * q.z;
+    r.x -= al * sv.x;
+    r.y -= al * sv.y;
+    r.z -= al * sv.z;
     const float4 w0 = d.wd0[i];
     const float2 w1 = d.wd1[i];
     const float zx = w0.x * r.x + w0.y * r.y + w0.z * r.z;
     const float zy = w0.y * r.x + w0.w * r.y + w1.x * r.z;
     const float zz = w0.z * r.x + w1.x * r.y + w1.y * r.z;
+    d.wp[i] = p;
+    d.ws[i] = sv;
     d.wx[i] = x;
     d.wr[i] = r;
-    d.wz[i] = make_float4(zx, zy, zz, 0.f);
+    d.wu[i] = make_float4(zx, zy, zz, 0.f);
     v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[1] = (double)r.x * (double)zx + (double)r.y * (double)zy + (double)r.z * (double)zz;
   }
@@ -262,16 +323,25 @@ __global__ void k_cg_update(Dev d) {
 }
 
 static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.P + block - 1) / block); }
-static inline dim3 grid4_of(const Dev& d, int block) { return dim3((4 * d.P + block - 1) / block); }
+// four-lane sweeps: a workgroup covers the particles of a one-lane workgroup
+// (fewer workgroups to dispatch when most scenes have left the solve)
+static inline int quad_block(int block) { return block * 4 <= 1024 ? block * 4 : 1024; }
+static inline dim3 grid4_of(const Dev& d, int block) {
+  const int qb = quad_block(block);
+  return dim3((4 * d.P + qb - 1) / qb);
+}
 
 void launch_strain(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_strain, grid_of(d, block), dim3(block), 0, st, d);
 }
 void launch_cg_init(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d);
 }
-void launch_cg_matvec(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_matvec, grid4_of(d, block), dim3(block), 0, st, d);
+void launch_cg_guess(const Dev& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d);
+}
+void launch_cg_matvec(const Dev& d, int block, int start, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_matvec, grid4_of(d, block), dim3(quad_block(block)), 0, st, d, start);
 }
 void launch_cg_update(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d);

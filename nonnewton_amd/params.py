@@ -86,7 +86,10 @@ class SceneParams:
     poisson_ratio: float = 0.33
     # <= 0 means volume0 * 1000 as in initialize_particles, SPH/main.py:743
     particle_mass: float = 0.0
-    warm_start: bool = False
+    # bit 0: the Weiler CG starts from v + (x - v) of the previous solve (the
+    # initial guess of the published solver; the reference declares its
+    # m_vDiff fields, SPH/main.py:1486-1487, with an empty solver around them)
+    warm_start: int = 1
     reserved: int = 0
 
     # ---- derived values, as the reference computes them in Python double ----

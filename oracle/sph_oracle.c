@@ -89,7 +89,8 @@ struct or_sim {
     /* non-Newtonian */
     real *gam, *eta;
     /* Weiler CG work vectors (fluid particles, 3 per particle) */
-    real *wx, *wr, *wz, *wp, *wq, *wdinv, *wmu, *wpair;
+    real *wx, *wr, *wz, *wp, *wq, *ws, *wdinv, *wmu, *wpair;
+    real *wvd; /* x - v of the previous solve (initial guess with warm_start bit 0) */
     int initialized;
 };
 
@@ -773,40 +774,73 @@ static void weiler_solve(or_sim *s, or_stats *st)
 {
     int n3 = 3 * s->n;
     for (int i = 0; i < s->n; i++) s->wmu[i] = is_fluid(s, i) ? mu_of(s, i) : (real)0.0;
-    /* b = v; x0 = b */
+    /* b = v; x0 = b, or with warm_start bit 0 x0 = b + theta e, e = (x - v) of
+     * the previous solve (the guess of SPlisHSPlasH's Viscosity_Weiler2018,
+     * whose m_vDiff fields the reference declares, SPH/main.py:1486-1487),
+     * scaled by theta = (e, r0) / (e, A e), r0 = b - A b: the theta that
+     * minimizes the A-norm error along e, so the start is never worse than
+     * x0 = b in that norm.  r = r0 - theta A e. */
     memcpy(s->wx, s->v, sizeof(real) * n3);
+    const int warm = s->p.warm_start & 1;
     weiler_precond(s);
-    weiler_apply(s, s->wx, s->wq);
+    weiler_apply(s, s->v, s->wq);
     for (int i = 0; i < n3; i++) s->wr[i] = s->v[i] - s->wq[i];
+    if (warm) {
+        weiler_apply(s, s->wvd, s->ws);
+        double num = fdot(s, s->wvd, s->wr), den = fdot(s, s->wvd, s->ws);
+        real th = (real)(den > 0.0 ? num / den : 0.0);
+        for (int i = 0; i < s->n; i++) {
+            if (!is_fluid(s, i)) continue;
+            for (int q = 0; q < 3; q++) {
+                int k = 3 * i + q;
+                s->wx[k] = s->v[k] + th * s->wvd[k];
+                s->wr[k] = s->wr[k] - th * s->ws[k];
+            }
+        }
+    }
     double bb = fdot(s, s->v, s->v);
     double rr = fdot(s, s->wr, s->wr);
     double tol2 = s->p.weiler_tol * s->p.weiler_tol * bb;
     int it = 0;
+    /* Preconditioned CG in the Chronopoulos-Gear arrangement (one reduction of
+     * three dot products per iteration); equal to textbook PCG in exact
+     * arithmetic for the symmetric operator:
+     *   u = M r, w = A u, gamma = (r, u), delta = (u, w)
+     *   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+     *   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s
+     * wz holds u, wq holds w, ws holds s = A p. */
     if (bb > 0.0 && rr > tol2) {
         weiler_precond_apply(s, s->wr, s->wz);
-        memcpy(s->wp, s->wz, sizeof(real) * n3);
-        double rz = fdot(s, s->wr, s->wz);
+        double ga = fdot(s, s->wr, s->wz);
+        weiler_apply(s, s->wz, s->wq);
+        double de = fdot(s, s->wz, s->wq);
+        double al = ga / de, be = 0.0;
         for (it = 1; it <= s->p.weiler_max_iter; it++) {
-            weiler_apply(s, s->wp, s->wq);
-            double pq = fdot(s, s->wp, s->wq);
-            real al = (real)(rz / pq);
+            real alf = (real)al, bef = (real)be;
             for (int i = 0; i < s->n; i++) {
                 if (!is_fluid(s, i)) continue;
                 for (int q = 0; q < 3; q++) {
-                    s->wx[3 * i + q] += al * s->wp[3 * i + q];
-                    s->wr[3 * i + q] -= al * s->wq[3 * i + q];
+                    int k = 3 * i + q;
+                    if (it == 1) {
+                        s->wp[k] = s->wz[k];
+                        s->ws[k] = s->wq[k];
+                    } else {
+                        s->wp[k] = s->wz[k] + bef * s->wp[k];
+                        s->ws[k] = s->wq[k] + bef * s->ws[k];
+                    }
+                    s->wx[k] += alf * s->wp[k];
+                    s->wr[k] -= alf * s->ws[k];
                 }
             }
+            weiler_precond_apply(s, s->wr, s->wz);
             rr = fdot(s, s->wr, s->wr);
             if (rr <= tol2) break;
-            weiler_precond_apply(s, s->wr, s->wz);
-            double rzn = fdot(s, s->wr, s->wz);
-            real be = (real)(rzn / rz);
-            rz = rzn;
-            for (int i = 0; i < s->n; i++) {
-                if (!is_fluid(s, i)) continue;
-                for (int q = 0; q < 3; q++) s->wp[3 * i + q] = s->wz[3 * i + q] + be * s->wp[3 * i + q];
-            }
+            double gn = fdot(s, s->wr, s->wz);
+            weiler_apply(s, s->wz, s->wq);
+            de = fdot(s, s->wz, s->wq);
+            be = gn / ga;
+            al = gn / (de - be * gn / al);
+            ga = gn;
         }
         if (it > s->p.weiler_max_iter) it = s->p.weiler_max_iter;
     }
@@ -814,7 +848,11 @@ static void weiler_solve(or_sim *s, or_stats *st)
     st->cg_rel_residual = bb > 0.0 ? sqrt(rr / bb) : 0.0;
     for (int i = 0; i < s->n; i++) {
         if (!is_fluid(s, i)) continue;
-        for (int q = 0; q < 3; q++) s->a[3 * i + q] += (s->wx[3 * i + q] - s->v[3 * i + q]) / s->dt;
+        for (int q = 0; q < 3; q++) {
+            real dv = s->wx[3 * i + q] - s->v[3 * i + q];
+            s->a[3 * i + q] += dv / s->dt;
+            if (warm) s->wvd[3 * i + q] = dv;
+        }
     }
 }
 
@@ -1148,6 +1186,8 @@ or_sim *or_create(const or_params *p, int32_t n, const float *x, const float *v,
     s->wz = (real *)calloc(n3, sizeof(real));
     s->wp = (real *)calloc(n3, sizeof(real));
     s->wq = (real *)calloc(n3, sizeof(real));
+    s->ws = (real *)calloc(n3, sizeof(real));
+    s->wvd = (real *)calloc(n3, sizeof(real));
     s->wdinv = (real *)calloc((size_t)6 * n, sizeof(real));
     s->wmu = (real *)calloc(n, sizeof(real));
     s->wpair = (real *)calloc((size_t)n * MAX_NB, sizeof(real));
@@ -1177,7 +1217,7 @@ void or_destroy(or_sim *s)
 {
     if (!s) return;
     void *ptrs[] = {s->x, s->v, s->a, s->x0, s->vol, s->rho, s->alpha, s->dadv, s->rest_vol, s->gam, s->eta,
-                    s->wx, s->wr, s->wz, s->wp, s->wq, s->wdinv, s->wmu, s->wpair, s->kind, s->s_of, s->gcount, s->gslot,
+                    s->wx, s->wr, s->wz, s->wp, s->wq, s->ws, s->wvd, s->wdinv, s->wmu, s->wpair, s->kind, s->s_of, s->gcount, s->gslot,
                     s->nbr, s->nnbr, s->s_pid, s->rest, s->rest_cnt, s->R, s->stress, s->Fm};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(s);

@@ -58,7 +58,7 @@ typedef struct or_params {
     double youngs_modulus;
     double poisson_ratio;
     double particle_mass;         /* <= 0: volume0 * 1000 (:743) */
-    int32_t warm_start;           /* reserved, 0 */
+    int32_t warm_start;           /* bit 0: Weiler CG starts from v + (x - v) of the previous solve */
     int32_t reserved;
 } or_params;
 

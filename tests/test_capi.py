@@ -25,7 +25,8 @@ def test_version_and_kernel_names():
     lib = L.load()
     assert lib.nn_version() == 1
     names = L.kernel_names()
-    assert names[0] == "cell_count" and names[-1] == "advect" and len(set(names)) == L.KERNEL_COUNT
+    assert names[0] == "cell_count" and names[-1] == "cg_guess" and len(set(names)) == L.KERNEL_COUNT
+    assert lib.nn_kernel_name(L.KERNEL_COUNT) == b""
 
 
 def test_struct_layout_matches_header(tmp_path):
@@ -57,7 +58,7 @@ def test_argument_errors_without_gpu():
     assert "positive" in L.last_error()
     bad = L.params_struct(SceneParams(viscosity_method=7))
     assert lib.nn_create(ctypes.byref(bad), 10, ctypes.byref(h)) == L.NN_E_ARG
-    bad = L.params_struct(SceneParams(warm_start=True))
+    bad = L.params_struct(SceneParams(warm_start=2))
     assert lib.nn_create(ctypes.byref(bad), 10, ctypes.byref(h)) == L.NN_E_UNSUPPORTED
     assert lib.nn_pool_footprint(1000, 0, ctypes.byref(L.params_struct(SceneParams()))) > 1000 * 500
 

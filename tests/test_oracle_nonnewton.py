@@ -66,10 +66,10 @@ def test_strain_rate_of_linear_field():
     assert s.get(O.F_STRAIN_NORM)[c] == pytest.approx(np.linalg.norm(sym), rel=0.03)
 
 
-def weiler_scene(v, eta0=5.0, method=2):
+def weiler_scene(v, eta0=5.0, method=2, warm=1):
     pts = lattice(4, origin=0.3) + np.random.default_rng(1).uniform(-2e-4, 2e-4, (64, 3))
     p = SceneParams(domain_end=(1, 1, 1), gravity=(0, 0, 0), dt=5e-4, use_non_newtonian=True,
-                    viscosity_method=VISCOSITY_WEILER, weiler_tol=1e-6, weiler_max_iter=500,
+                    viscosity_method=VISCOSITY_WEILER, weiler_tol=1e-6, weiler_max_iter=500, warm_start=warm,
                     **{**NN, "viscosity0": eta0, "non_newton_method": method})
     return O.OracleSim(p, pts, np.zeros(len(pts), np.int32), v=v, precision="f64")
 
@@ -95,3 +95,36 @@ def test_weiler_leaves_uniform_motion_alone():
     A, b, x, st = s.weiler_probe()
     assert st["iterations_cg"] == 0
     assert np.array_equal(x, b)
+
+
+def test_weiler_warm_start_converges_to_the_same_solution():
+    """The scaled guess v + theta (x - v)_prev moves where CG starts, not what it
+    converges to: after two steps the warm solve meets ||A x - b|| <= tol ||b||
+    and agrees with the dense solution of its system."""
+    v = np.random.default_rng(5).normal(0, 0.1, (64, 3))
+    s = weiler_scene(v, eta0=20.0, warm=1)
+    s.init()
+    s.step()
+    s.step()
+    A, b, x, st = s.weiler_probe()
+    exact = np.linalg.solve(A, b)
+    assert st["iterations_cg"] > 0
+    assert np.linalg.norm(A @ x - b) <= 1.01e-6 * np.linalg.norm(b)
+    assert np.linalg.norm(x - exact) <= 1e-4 * np.linalg.norm(exact)
+
+
+def test_weiler_warm_start_saves_iterations_in_a_dam_break():
+    """In a gravity-driven flow the viscous velocity change varies slowly from
+    step to step: started from the previous one, CG needs markedly fewer
+    iterations for the same tolerance than started from x0 = v."""
+    import dataclasses
+    from nonnewton_amd import scenes
+    sc0 = scenes.mixed_scene(np.random.Generator(np.random.PCG64(24)), 2500, 1, 14.0, 1.0, False, "m")
+    its = {}
+    for warm in (0, 1):
+        sc = dataclasses.replace(sc0, params=sc0.params.replace(warm_start=warm))
+        s = O.OracleSim.from_scene(sc, precision="f32")
+        s.init()
+        its[warm] = [s.step()["iterations_cg"] for _ in range(8)][2:]
+    assert its[0][0] > 10
+    assert sum(its[1]) < 0.75 * sum(its[0]), its

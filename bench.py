@@ -106,8 +106,10 @@ def load_workload(name):
     return [sc], f"{name}: {sc.n} particles ({sc.n_dynamic} dynamic), dt {sc.params.dt}"
 
 
-def run_batched(scs, steps, warmup, profile=True):
+def run_batched(scs, steps, warmup, profile=True, block=0):
     pool = L.Pool(200 << 30)
+    if block:
+        pool.set_block(block)
     t0 = time.time()
     slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in scs]
     if any(q is None for q in slots):
@@ -140,10 +142,12 @@ def run_batched(scs, steps, warmup, profile=True):
     return elapsed, lat, per_step_stats, times, breakdown
 
 
-def run_plain(scs, steps, warmup):
+def run_plain(scs, steps, warmup, block=0):
     ctxs = []
     for s in scs:
         c = L.Context(s.params, s.n)
+        if block:
+            c.set_block(block)
         c.upload(s.x, s.kind, s.v)
         c.init()
         for k in range(warmup):
@@ -216,6 +220,31 @@ def cpu_baseline(scs, budget_s):
     }
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: each launch reads and writes
+    mib MiB, so FETCH_SIZE and WRITE_SIZE of its dispatches (KiB) give the
+    counters' scale for HBM bytes.  Uses the HIP runtime directly (hipMemcpy
+    device to device runs the runtime's copy kernel)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib * (1 << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(5):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 5}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,7 +258,13 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 5 times (calibrates the rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE counters against a kernel of known traffic) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SPH scenes are independent and shard across GPUs without exchange; "
               "run it with --gpus 1", file=sys.stderr)
@@ -240,7 +275,7 @@ def main():
     n_all = sum(s.n for s in scs)
     log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
 
-    elapsed, lat, stats, ktimes, breakdown = run_batched(scs, args.steps, args.warmup)
+    elapsed, lat, stats, ktimes, breakdown = run_batched(scs, args.steps, args.warmup, block=args.block)
     value = n_dyn * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] batched: {ms_per_step:.3f} ms/tick, {value:.4g} particle-steps/s")
@@ -248,7 +283,7 @@ def main():
     plain_value = None
     plain_lat = []
     if not args.no_plain:
-        pe, plain_lat = run_plain(scs, args.plain_steps, 1)
+        pe, plain_lat = run_plain(scs, args.plain_steps, 1, block=args.block)
         plain_value = n_dyn * args.plain_steps / pe
         log(f"[bench] plain: {1e3 * pe / args.plain_steps:.3f} ms per round of {len(scs)} scenes, "
             f"{plain_value:.4g} particle-steps/s")

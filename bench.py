@@ -106,7 +106,7 @@ def load_workload(name):
     return [sc], f"{name}: {sc.n} particles ({sc.n_dynamic} dynamic), dt {sc.params.dt}"
 
 
-def run_batched(scs, steps, warmup, profile=True, block=0):
+def run_batched(scs, steps, warmup, profile=True, block=0, tune=True):
     pool = L.Pool(200 << 30)
     if block:
         pool.set_block(block)
@@ -117,6 +117,11 @@ def run_batched(scs, steps, warmup, profile=True, block=0):
     log(f"[bench] admitted {len(slots)} scenes in {time.time() - t0:.2f} s")
     for k in range(warmup):
         pool.tick()
+    shapes = {}
+    if tune:
+        from nonnewton_amd import autotune
+        shapes = autotune.tune_pool(pool, len(scs), sum(s.n for s in scs), use_cache=False)
+        log(f"[bench] launch shapes: {shapes}")
     breakdown = {}
     dominant = None
     if profile:
@@ -139,15 +144,17 @@ def run_batched(scs, steps, warmup, profile=True, block=0):
     times = {dominant: pool.kernel_times()[dominant]} if profile else {}
     pool.profile(False)
     pool.close()
-    return elapsed, lat, per_step_stats, times, breakdown
+    return elapsed, lat, per_step_stats, times, breakdown, shapes
 
 
-def run_plain(scs, steps, warmup, block=0):
+def run_plain(scs, steps, warmup, block=0, shapes=None):
     ctxs = []
     for s in scs:
         c = L.Context(s.params, s.n)
         if block:
             c.set_block(block)
+        for k, b in (shapes or {}).items():
+            c.set_kernel_block(k, b)
         c.upload(s.x, s.kind, s.v)
         c.init()
         for k in range(warmup):
@@ -256,6 +263,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -275,7 +283,8 @@ def main():
     n_all = sum(s.n for s in scs)
     log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
 
-    elapsed, lat, stats, ktimes, breakdown = run_batched(scs, args.steps, args.warmup, block=args.block)
+    elapsed, lat, stats, ktimes, breakdown, shapes = run_batched(scs, args.steps, args.warmup, block=args.block,
+                                                                 tune=not args.no_tune)
     value = n_dyn * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] batched: {ms_per_step:.3f} ms/tick, {value:.4g} particle-steps/s")
@@ -283,7 +292,7 @@ def main():
     plain_value = None
     plain_lat = []
     if not args.no_plain:
-        pe, plain_lat = run_plain(scs, args.plain_steps, 1, block=args.block)
+        pe, plain_lat = run_plain(scs, args.plain_steps, 1, block=args.block, shapes=shapes)
         plain_value = n_dyn * args.plain_steps / pe
         log(f"[bench] plain: {1e3 * pe / args.plain_steps:.3f} ms per round of {len(scs)} scenes, "
             f"{plain_value:.4g} particle-steps/s")
@@ -322,6 +331,7 @@ def main():
         "p50_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 50)), 3),
         "p99_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 99)), 3),
         "mean_iterations_v_p_cg": [round(float(x), 2) for x in it.mean(axis=0)],
+        "launch_shapes": shapes,
     }
     if plain_value is not None:
         cfg["plain_particle_steps_per_s"] = round(plain_value, 1)

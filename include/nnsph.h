@@ -198,6 +198,11 @@ int32_t nn_pool_capacity(void);
  * the autotuner sets it.  0 = library default. */
 int32_t nn_pool_set_block(nn_pool *pool, int32_t block);
 int32_t nn_set_block(nn_ctx *ctx, int32_t block);
+/* Workgroup size of one kernel (ids of nn_kernel_name; 0 = the default above).
+ * The sweeps' results do not depend on it: per-particle arithmetic and the
+ * grouping of the partial sums are fixed by the wavefront, not the workgroup. */
+int32_t nn_pool_set_kernel_block(nn_pool *pool, int32_t kernel, int32_t block);
+int32_t nn_set_kernel_block(nn_ctx *ctx, int32_t kernel, int32_t block);
 
 /* Per-kernel timing with HIP events recorded around every launch on the stream
  * the kernel runs on (no counterpart in the reference).  enable != 0 turns it

@@ -146,6 +146,8 @@ def load():
         "nn_pool_set_block": (i32, [vp, i32]),
         "nn_kernel_name": (ctypes.c_char_p, [i32]),
         "nn_profile": (i32, [vp, i32]),
+        "nn_set_kernel_block": (i32, [vp, i32, i32]),
+        "nn_pool_set_kernel_block": (i32, [vp, i32, i32]),
         "nn_profile_mask": (i32, [vp, ctypes.c_uint32]),
         "nn_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
         "nn_pool_profile": (i32, [vp, i32]),
@@ -161,6 +163,16 @@ def load():
 
 
 KERNEL_COUNT = 21
+
+
+def device_name() -> str:
+    """Name of the device the library runs on (nn_device_info)."""
+    lib = load()
+    buf = ctypes.create_string_buffer(256)
+    hbm = ctypes.c_int64()
+    cus = ctypes.c_int32()
+    check(lib.nn_device_info(buf, 256, ctypes.byref(hbm), ctypes.byref(cus)), "nn_device_info")
+    return buf.value.decode(errors="replace")
 
 
 def kernel_mask(kernels) -> int:
@@ -278,6 +290,9 @@ class Context:
     def set_block(self, block: int):
         check(self.lib.nn_set_block(self.h, int(block)), "nn_set_block")
 
+    def set_kernel_block(self, kernel: str, block: int):
+        check(self.lib.nn_set_kernel_block(self.h, kernel_names().index(kernel), int(block)), "nn_set_kernel_block")
+
     def profile(self, enable: bool = True, kernels=None):
         """Time every kernel (kernels None) or only the named ones."""
         if kernels is None:
@@ -370,6 +385,10 @@ class Pool:
 
     def set_block(self, block: int):
         check(self.lib.nn_pool_set_block(self.h, int(block)), "nn_pool_set_block")
+
+    def set_kernel_block(self, kernel: str, block: int):
+        check(self.lib.nn_pool_set_kernel_block(self.h, kernel_names().index(kernel), int(block)),
+              "nn_pool_set_kernel_block")
 
     def profile(self, enable: bool = True, kernels=None):
         """Time every kernel (kernels None) or only the named ones."""

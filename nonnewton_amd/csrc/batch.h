@@ -89,7 +89,9 @@ class Batch {
   int refresh_scene_table();  // after a parameter change
   int synchronize();
   long long bytes() const { return bytes_; }
-  int block = 256;
+  int block = 256;             // default workgroup size of the sweeps
+  int kblock[K_COUNT] = {};    // per-kernel workgroup size (0 = block), set by the autotuner
+  int blk(int id) const { return kblock[id] ? kblock[id] : block; }
   hipStream_t stream;
   Dev d{};
   KernelTimer timer;

@@ -511,12 +511,12 @@ void KernelTimer::reset() {
 
 int Batch::search_impl() {
   launch_fill_i32(d.cell_count, (long)C_total_ + 1, 0, stream);
-  TIMED(K_CELL_COUNT, launch_cell_count(d, block, stream));
+  TIMED(K_CELL_COUNT, launch_cell_count(d, blk(K_CELL_COUNT), stream));
   TIMED(K_SCAN, launch_scan(d, C_total_ + 1, stream));
-  TIMED(K_BUCKET, launch_bucket(d, block, stream));
-  TIMED(K_PLACE, launch_place(d, block, stream));
+  TIMED(K_BUCKET, launch_bucket(d, blk(K_BUCKET), stream));
+  TIMED(K_PLACE, launch_place(d, blk(K_PLACE), stream));
   swap_sort_buffers();
-  TIMED(K_BUILD_LISTS, launch_build_lists(d, block, stream));
+  TIMED(K_BUILD_LISTS, launch_build_lists(d, blk(K_BUILD_LISTS), stream));
   return NN_OK;
 }
 
@@ -655,16 +655,16 @@ int Batch::solver_loop(int which) {
   for (;;) {
     for (int c = 0; c < chunk; c++) {
       if (which == 0) {
-        TIMED(K_DIV_UPDATE, launch_div_update(d, block, stream));
-        TIMED(K_DIV_SOURCE, launch_div_source(d, block, stream));
+        TIMED(K_DIV_UPDATE, launch_div_update(d, blk(K_DIV_UPDATE), stream));
+        TIMED(K_DIV_SOURCE, launch_div_source(d, blk(K_DIV_SOURCE), stream));
         launch_scene_finish(d, 0, stream);
       } else if (which == 1) {
-        TIMED(K_PRESS_UPDATE, launch_press_update(d, block, stream));
-        TIMED(K_PRESS_SOURCE, launch_press_source(d, block, 1, stream));
+        TIMED(K_PRESS_UPDATE, launch_press_update(d, blk(K_PRESS_UPDATE), stream));
+        TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), 1, stream));
         launch_scene_finish(d, 1, stream);
       } else {
-        TIMED(K_CG_UPDATE, launch_cg_update(d, block, stream));
-        TIMED(K_CG_MATVEC, launch_cg_matvec(d, block, 0, stream));
+        TIMED(K_CG_UPDATE, launch_cg_update(d, blk(K_CG_UPDATE), stream));
+        TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), 0, stream));
         launch_scene_finish(d, 3, stream);
       }
     }
@@ -687,35 +687,35 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
     }
   launch_step_begin(d, stream);
   CHK(search_impl());                                                  // :917
-  if (any_elastic_) TIMED(K_BVOL, launch_bvol(d, block, 1, stream));  // :918
-  TIMED(K_CLEAR_ACC, launch_clear_acc(d, block, stream));             // :920
+  if (any_elastic_) TIMED(K_BVOL, launch_bvol(d, blk(K_BVOL), 1, stream));  // :918
+  TIMED(K_CLEAR_ACC, launch_clear_acc(d, blk(K_CLEAR_ACC), stream));             // :920
   if (any_fluid_) {  // DFSPHSolver.substep, :1546-1555
-    TIMED(K_DENSITY_ALPHA, launch_density_alpha(d, block, stream));
+    TIMED(K_DENSITY_ALPHA, launch_density_alpha(d, blk(K_DENSITY_ALPHA), stream));
     if (any_div_) {
       launch_solver_begin(d, 0, stream);
       CHK(solver_loop(0));
     }
-    if (any_strain_) TIMED(K_STRAIN, launch_strain(d, block, stream));
+    if (any_strain_) TIMED(K_STRAIN, launch_strain(d, blk(K_STRAIN), stream));
     if (any_weiler_) {
       launch_solver_begin(d, 2, stream);
-      TIMED(K_CG_INIT, launch_cg_init(d, block, stream));
+      TIMED(K_CG_INIT, launch_cg_init(d, blk(K_CG_INIT), stream));
       if (any_warm_cg_) launch_scene_finish(d, 4, stream);
-      TIMED(K_CG_GUESS, launch_cg_guess(d, block, stream));
-      TIMED(K_CG_MATVEC, launch_cg_matvec(d, block, 1, stream));
+      TIMED(K_CG_GUESS, launch_cg_guess(d, blk(K_CG_GUESS), stream));
+      TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), 1, stream));
       launch_scene_finish(d, 2, stream);
       CHK(solver_loop(2));
     }
-    TIMED(K_NONPRESSURE, launch_nonpressure(d, block, stream));
+    TIMED(K_NONPRESSURE, launch_nonpressure(d, blk(K_NONPRESSURE), stream));
     std::swap(d.vel, d.vel_alt);
     launch_solver_begin(d, 1, stream);
-    TIMED(K_PRESS_SOURCE, launch_press_source(d, block, 0, stream));
+    TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), 0, stream));
     CHK(solver_loop(1));
   }
   if (any_elastic_) {  // Elasticity.substep, :1241-1244
-    TIMED(K_ELASTIC_ROT_STRESS, launch_elastic_rot_stress(d, block, stream));
-    TIMED(K_ELASTIC_FORCE, launch_elastic_force(d, block, stream));
+    TIMED(K_ELASTIC_ROT_STRESS, launch_elastic_rot_stress(d, blk(K_ELASTIC_ROT_STRESS), stream));
+    TIMED(K_ELASTIC_FORCE, launch_elastic_force(d, blk(K_ELASTIC_FORCE), stream));
   }
-  TIMED(K_ADVECT, launch_advect(d, block, stream));  // :933, :936-937
+  TIMED(K_ADVECT, launch_advect(d, blk(K_ADVECT), stream));  // :933, :936-937
   HIPCHK(hipGetLastError());
   CHK(gather_state());
   CHK(timer.collect());

@@ -105,12 +105,8 @@ __global__ void k_cg_init(Dev d) {
     const float mui = d.mu[i];
     const int cnt = d.nnbr[i];
     const long P = d.P;
-    for (int k = q; k < cnt; k += 4) {
-      const int j = d.nbr[k * P + i];
-      const float4 xj = d.pos[j];
-      const float4 vj = d.vel[j];
-      const float muj = d.mu[j];
-      const float rhoj = d.rho[j];
+    // one neighbor's contribution, in list order
+    auto pair_term = [&](int k, const float4& xj, const float4& vj, const float4& ej, float muj, float rhoj) {
       float sij = 0.0f;
       if (xj.w > 0.0f) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -120,7 +116,6 @@ __global__ void k_cg_init(Dev d) {
         ay += t * ry;
         az += t * rz;
         if (warm) {
-          const float4 ej = d.vdiff[j];
           const float te = sij * ((ei.x - ej.x) * rx + (ei.y - ej.y) * ry + (ei.z - ej.z) * rz);
           bx += te * rx;
           by += te * ry;
@@ -134,6 +129,21 @@ __global__ void k_cg_init(Dev d) {
         m5 += sij * rz * rz;
       }
       d.spair[k * P + i] = sij;
+    };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // two list positions of this lane per batch, their loads issued together
+    for (int k = q; k < cnt; k += 8) {
+      const bool two = k + 4 < cnt;
+      const int j0 = d.nbr[k * P + i];
+      const int j1 = two ? d.nbr[(k + 4) * P + i] : i;
+      const float4 x0 = d.pos[j0], x1 = d.pos[j1];
+      const float4 v0 = d.vel[j0], v1 = d.vel[j1];
+      const float mu0 = d.mu[j0], mu1 = d.mu[j1];
+      const float r0 = d.rho[j0], r1 = d.rho[j1];
+      const float4 e0 = warm ? d.vdiff[j0] : zero4;
+      const float4 e1 = warm ? d.vdiff[j1] : zero4;
+      pair_term(k, x0, v0, e0, mu0, r0);
+      if (two) pair_term(k + 4, x1, v1, e1, mu1, r1);
     }
   }
   ax = quad_sum(ax);

@@ -1,6 +1,8 @@
 """The batched/serving pool on the GPU: every scene's results are the same bit
 for bit alone and in the mix (including after retire/admit churn), equal the
 oracle, and admission honours the memory budget."""
+import json
+
 import numpy as np
 import pytest
 
@@ -42,6 +44,40 @@ def test_pool_equals_alone_and_churn():
         assert (ps["iterations_v"], ps["iterations_p"], ps["iterations_cg"]) == \
             (st["iterations_v"], st["iterations_p"], st["iterations_cg"])
     pool.close()
+
+
+def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
+    """Every kernel at 64-, 1024- and autotuned workgroups: the same bits as the
+    default shapes (the partial sums are grouped per wavefront, not per
+    workgroup)."""
+    from nonnewton_amd import autotune
+    monkeypatch.setenv("NONNEWTON_AMD_TUNE_CACHE", str(tmp_path / "shapes.json"))
+    batch = scenes.mixed_batch(count=3, lo=2000, hi=5000)
+    ref = []
+    for sc in batch:
+        out, _ = alone(sc, 6)
+        ref.append(out)
+    for mode in (64, 1024, "tuned"):
+        pool = L.Pool(8 << 30)
+        slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in batch]
+        if mode == "tuned":
+            shapes = autotune.tune_pool(pool, len(batch), sum(s.n for s in batch), ticks=1,
+                                        candidates=(64, 256, 1024))
+            assert shapes and all(b in (64, 256, 1024) for b in shapes.values())
+            steps_done = 6  # the tuner stepped the pool 2 x 3 x 1 times
+        else:
+            for k in L.kernel_names():
+                if k != "scan":
+                    pool.set_kernel_block(k, mode)
+            steps_done = 0
+        for _ in range(6 - steps_done):
+            pool.tick()
+        for k, slot in enumerate(slots):
+            for f, a in ref[k].items():
+                assert np.array_equal(pool.read(slot, f), a), (mode, k, f)
+        pool.close()
+    with open(tmp_path / "shapes.json") as fh:
+        assert len(json.load(fh)) == 1
 
 
 def test_pool_newtonian_scene_matches_oracle():

@@ -99,6 +99,21 @@ def scene_algorithmic_bytes(kernel, sc, st):
     return b
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/<kernel>_<workload>_traffic.md,
+    counters scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def load_workload(name):
     if name == "mixed_batch":
         return S.mixed_batch(), "mixed_batch: 64 scenes, dynamic particles log-uniform 2k-60k, seed 3, dt 5e-4"
@@ -120,7 +135,7 @@ def run_batched(scs, steps, warmup, profile=True, block=0, tune=True):
     shapes = {}
     if tune:
         from nonnewton_amd import autotune
-        shapes = autotune.tune_pool(pool, len(scs), sum(s.n for s in scs), use_cache=False)
+        shapes = autotune.tune_pool(pool, slots, [s.n_fluid for s in scs], use_cache=False)
         log(f"[bench] launch shapes: {shapes}")
     breakdown = {}
     dominant = None
@@ -153,8 +168,10 @@ def run_plain(scs, steps, warmup, block=0, shapes=None):
         c = L.Context(s.params, s.n)
         if block:
             c.set_block(block)
-        for k, b in (shapes or {}).items():
-            c.set_kernel_block(k, b)
+        for k, sh in (shapes or {}).items():  # the batched form's tuned shapes
+            c.set_kernel_block(k, sh["block"])
+            if sh.get("depth"):
+                c.set_kernel_depth(k, sh["depth"])
         c.upload(s.x, s.kind, s.v)
         c.init()
         for k in range(warmup):
@@ -313,7 +330,8 @@ def main():
         if ok and launches:
             achieved = total_bytes / (ms * 1e-3) / 1e9
             roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(name, args.workload),
                     "launches": launches, "avg_launch_us": round(1e3 * ms / launches, 2),
                     "bytes_per_launch": round(total_bytes / launches),
                     "share_of_timed_region": round(ms / (1e3 * elapsed), 3)}

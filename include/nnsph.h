@@ -203,6 +203,12 @@ int32_t nn_set_block(nn_ctx *ctx, int32_t block);
  * grouping of the partial sums are fixed by the wavefront, not the workgroup. */
 int32_t nn_pool_set_kernel_block(nn_pool *pool, int32_t kernel, int32_t block);
 int32_t nn_set_kernel_block(nn_ctx *ctx, int32_t kernel, int32_t block);
+/* Gather depth of one kernel: how many neighbor-list entries a lane loads
+ * before using them (0 = the kernel's default; 1..16, a power of two).  The
+ * neighbor sweeps take 4, 8 or 16 (default 8), cg_matvec 1, 2, 4 or 8 (default
+ * 2); other values select the default.  Results do not depend on it. */
+int32_t nn_pool_set_kernel_depth(nn_pool *pool, int32_t kernel, int32_t depth);
+int32_t nn_set_kernel_depth(nn_ctx *ctx, int32_t kernel, int32_t depth);
 
 /* Per-kernel timing with HIP events recorded around every launch on the stream
  * the kernel runs on (no counterpart in the reference).  enable != 0 turns it

@@ -147,6 +147,8 @@ def load():
         "nn_kernel_name": (ctypes.c_char_p, [i32]),
         "nn_profile": (i32, [vp, i32]),
         "nn_set_kernel_block": (i32, [vp, i32, i32]),
+        "nn_set_kernel_depth": (i32, [vp, i32, i32]),
+        "nn_pool_set_kernel_depth": (i32, [vp, i32, i32]),
         "nn_pool_set_kernel_block": (i32, [vp, i32, i32]),
         "nn_profile_mask": (i32, [vp, ctypes.c_uint32]),
         "nn_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
@@ -293,6 +295,9 @@ class Context:
     def set_kernel_block(self, kernel: str, block: int):
         check(self.lib.nn_set_kernel_block(self.h, kernel_names().index(kernel), int(block)), "nn_set_kernel_block")
 
+    def set_kernel_depth(self, kernel: str, depth: int):
+        check(self.lib.nn_set_kernel_depth(self.h, kernel_names().index(kernel), int(depth)), "nn_set_kernel_depth")
+
     def profile(self, enable: bool = True, kernels=None):
         """Time every kernel (kernels None) or only the named ones."""
         if kernels is None:
@@ -389,6 +394,10 @@ class Pool:
     def set_kernel_block(self, kernel: str, block: int):
         check(self.lib.nn_pool_set_kernel_block(self.h, kernel_names().index(kernel), int(block)),
               "nn_pool_set_kernel_block")
+
+    def set_kernel_depth(self, kernel: str, depth: int):
+        check(self.lib.nn_pool_set_kernel_depth(self.h, kernel_names().index(kernel), int(depth)),
+              "nn_pool_set_kernel_depth")
 
     def profile(self, enable: bool = True, kernels=None):
         """Time every kernel (kernels None) or only the named ones."""

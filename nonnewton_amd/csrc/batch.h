@@ -91,7 +91,9 @@ class Batch {
   long long bytes() const { return bytes_; }
   int block = 256;             // default workgroup size of the sweeps
   int kblock[K_COUNT] = {};    // per-kernel workgroup size (0 = block), set by the autotuner
+  int kdepth[K_COUNT] = {};    // per-kernel gather depth (0 = the kernel's default), set by the autotuner
   int blk(int id) const { return kblock[id] ? kblock[id] : block; }
+  int dep(int id) const { return kdepth[id]; }
   hipStream_t stream;
   Dev d{};
   KernelTimer timer;

@@ -655,16 +655,16 @@ int Batch::solver_loop(int which) {
   for (;;) {
     for (int c = 0; c < chunk; c++) {
       if (which == 0) {
-        TIMED(K_DIV_UPDATE, launch_div_update(d, blk(K_DIV_UPDATE), stream));
-        TIMED(K_DIV_SOURCE, launch_div_source(d, blk(K_DIV_SOURCE), stream));
+        TIMED(K_DIV_UPDATE, launch_div_update(d, blk(K_DIV_UPDATE), dep(K_DIV_UPDATE), stream));
+        TIMED(K_DIV_SOURCE, launch_div_source(d, blk(K_DIV_SOURCE), dep(K_DIV_SOURCE), stream));
         launch_scene_finish(d, 0, stream);
       } else if (which == 1) {
-        TIMED(K_PRESS_UPDATE, launch_press_update(d, blk(K_PRESS_UPDATE), stream));
-        TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), 1, stream));
+        TIMED(K_PRESS_UPDATE, launch_press_update(d, blk(K_PRESS_UPDATE), dep(K_PRESS_UPDATE), stream));
+        TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 1, stream));
         launch_scene_finish(d, 1, stream);
       } else {
         TIMED(K_CG_UPDATE, launch_cg_update(d, blk(K_CG_UPDATE), stream));
-        TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), 0, stream));
+        TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
         launch_scene_finish(d, 3, stream);
       }
     }
@@ -690,25 +690,25 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
   if (any_elastic_) TIMED(K_BVOL, launch_bvol(d, blk(K_BVOL), 1, stream));  // :918
   TIMED(K_CLEAR_ACC, launch_clear_acc(d, blk(K_CLEAR_ACC), stream));             // :920
   if (any_fluid_) {  // DFSPHSolver.substep, :1546-1555
-    TIMED(K_DENSITY_ALPHA, launch_density_alpha(d, blk(K_DENSITY_ALPHA), stream));
+    TIMED(K_DENSITY_ALPHA, launch_density_alpha(d, blk(K_DENSITY_ALPHA), dep(K_DENSITY_ALPHA), stream));
     if (any_div_) {
       launch_solver_begin(d, 0, stream);
       CHK(solver_loop(0));
     }
-    if (any_strain_) TIMED(K_STRAIN, launch_strain(d, blk(K_STRAIN), stream));
+    if (any_strain_) TIMED(K_STRAIN, launch_strain(d, blk(K_STRAIN), dep(K_STRAIN), stream));
     if (any_weiler_) {
       launch_solver_begin(d, 2, stream);
       TIMED(K_CG_INIT, launch_cg_init(d, blk(K_CG_INIT), stream));
       if (any_warm_cg_) launch_scene_finish(d, 4, stream);
       TIMED(K_CG_GUESS, launch_cg_guess(d, blk(K_CG_GUESS), stream));
-      TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), 1, stream));
+      TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), dep(K_CG_MATVEC), 1, stream));
       launch_scene_finish(d, 2, stream);
       CHK(solver_loop(2));
     }
-    TIMED(K_NONPRESSURE, launch_nonpressure(d, blk(K_NONPRESSURE), stream));
+    TIMED(K_NONPRESSURE, launch_nonpressure(d, blk(K_NONPRESSURE), dep(K_NONPRESSURE), stream));
     std::swap(d.vel, d.vel_alt);
     launch_solver_begin(d, 1, stream);
-    TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), 0, stream));
+    TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 0, stream));
     CHK(solver_loop(1));
   }
   if (any_elastic_) {  // Elasticity.substep, :1241-1244

@@ -252,6 +252,14 @@ int32_t nn_set_block(nn_ctx* ctx, int32_t block) {
   return NN_OK;
 }
 
+static bool depth_ok(int32_t depth) { return depth == 0 || (depth >= 1 && depth <= 16 && !(depth & (depth - 1))); }
+
+int32_t nn_set_kernel_depth(nn_ctx* ctx, int32_t kernel, int32_t depth) {
+  if (!ctx || kernel < 0 || kernel >= nn::K_COUNT || !depth_ok(depth)) return NN_E_ARG;
+  ctx->batch->kdepth[kernel] = depth;
+  return NN_OK;
+}
+
 int32_t nn_set_kernel_block(nn_ctx* ctx, int32_t kernel, int32_t block) {
   if (!ctx || kernel < 0 || kernel >= nn::K_COUNT) return NN_E_ARG;
   if (block != 0 && (block < 64 || block > 1024 || (block & (block - 1)))) return NN_E_ARG;
@@ -284,6 +292,7 @@ struct nn_pool {
   std::mutex mu;
   int block = 256;
   int kblock[nn::K_COUNT] = {};
+  int kdepth[nn::K_COUNT] = {};
   std::vector<int> member_slot;  // batch index -> slot
 };
 
@@ -384,6 +393,7 @@ int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const
   std::unique_ptr<Batch> stg(new Batch(ps.staging_stream));
   stg->block = p->block;
   std::copy(p->kblock, p->kblock + nn::K_COUNT, stg->kblock);
+  std::copy(p->kdepth, p->kdepth + nn::K_COUNT, stg->kdepth);
   stg->scenes = {sc};
   rc = stg->layout({});
   if (rc < 0) return fail(rc);
@@ -433,6 +443,7 @@ static int pool_apply_membership(nn_pool* p) {
   if (!changed) return NN_OK;
   p->main->block = p->block;
   std::copy(p->kblock, p->kblock + nn::K_COUNT, p->main->kblock);
+  std::copy(p->kdepth, p->kdepth + nn::K_COUNT, p->main->kdepth);
   p->main->scenes = members;
   int rc = p->main->layout(from);
   if (rc < 0) return rc;
@@ -540,6 +551,14 @@ int32_t nn_pool_set_block(nn_pool* p, int32_t block) {
   std::lock_guard<std::mutex> lk(p->mu);
   p->block = block;
   p->main->block = block;
+  return NN_OK;
+}
+
+int32_t nn_pool_set_kernel_depth(nn_pool* p, int32_t kernel, int32_t depth) {
+  if (!p || kernel < 0 || kernel >= nn::K_COUNT || !depth_ok(depth)) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->kdepth[kernel] = depth;
+  p->main->kdepth[kernel] = depth;
   return NN_OK;
 }
 

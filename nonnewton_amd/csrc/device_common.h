@@ -89,6 +89,18 @@ __device__ __forceinline__ void for_neighbors(const Dev& d, int i, int cnt, Load
   }
 }
 
+// Launch a sweep templated on its batch size B of for_neighbors (4, 8 or 16;
+// anything else is the default 8).  B changes how many gathers are in flight,
+// not the order of any sum.
+#define DEPTH_LAUNCH(kern, depth, grid, blk, st, ...)                          \
+  do {                                                                        \
+    switch (depth) {                                                          \
+      case 4: hipLaunchKernelGGL(kern<4>, grid, blk, 0, st, __VA_ARGS__); break; \
+      case 16: hipLaunchKernelGGL(kern<16>, grid, blk, 0, st, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL(kern<8>, grid, blk, 0, st, __VA_ARGS__); break; \
+    }                                                                         \
+  } while (0)
+
 struct PosVel {
   float4 x, v;
 };

@@ -149,18 +149,18 @@ void launch_place(const Dev& d, int block, hipStream_t st);
 void launch_build_lists(const Dev& d, int block, hipStream_t st);
 void launch_bvol(const Dev& d, int block, int dynamic, hipStream_t st);
 
-void launch_density_alpha(const Dev& d, int block, hipStream_t st);
-void launch_div_update(const Dev& d, int block, hipStream_t st);
-void launch_div_source(const Dev& d, int block, hipStream_t st);
+void launch_density_alpha(const Dev& d, int block, int depth, hipStream_t st);
+void launch_div_update(const Dev& d, int block, int depth, hipStream_t st);
+void launch_div_source(const Dev& d, int block, int depth, hipStream_t st);
 void launch_solver_begin(const Dev& d, int which, hipStream_t st);
-void launch_strain(const Dev& d, int block, hipStream_t st);
+void launch_strain(const Dev& d, int block, int depth, hipStream_t st);
 void launch_cg_init(const Dev& d, int block, hipStream_t st);
 void launch_cg_guess(const Dev& d, int block, hipStream_t st);
-void launch_cg_matvec(const Dev& d, int block, int start, hipStream_t st);
+void launch_cg_matvec(const Dev& d, int block, int depth, int start, hipStream_t st);  // depth 1, 2, 4, 8
 void launch_cg_update(const Dev& d, int block, hipStream_t st);
-void launch_nonpressure(const Dev& d, int block, hipStream_t st);
-void launch_press_source(const Dev& d, int block, int reduce, hipStream_t st);
-void launch_press_update(const Dev& d, int block, hipStream_t st);
+void launch_nonpressure(const Dev& d, int block, int depth, hipStream_t st);
+void launch_press_source(const Dev& d, int block, int depth, int reduce, hipStream_t st);
+void launch_press_update(const Dev& d, int block, int depth, hipStream_t st);
 void launch_clear_acc(const Dev& d, int block, hipStream_t st);
 void launch_elastic_init(const Dev& d, int block, hipStream_t st);
 void launch_elastic_rot_stress(const Dev& d, int block, hipStream_t st);

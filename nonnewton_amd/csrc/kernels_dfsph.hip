@@ -41,6 +41,7 @@ __global__ void k_clear_acc(Dev d) {
 // compute_densities (:962-984), compute_DFSPH_factor (:1569-1608) and the first
 // compute_density_change of the divergence solve (:1610-1645), fused: all three
 // read the same neighbors and none reads what another writes.
+template <int B>
 __global__ void k_density_alpha(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -55,7 +56,7 @@ __global__ void k_density_alpha(Dev d) {
     KSum den{0.0f, 0.0f};
     float denp = 0.0f;
     float gix = 0.f, giy = 0.f, giz = 0.f, sk = 0.f, rd = 0.f;
-    for_neighbors<8>(
+    for_neighbors<B>(
         d, i, cnt, [&](int j) { return PosVel{d.pos[j], d.vel[j]}; },
         [&](int, const PosVel& nj) {
           const float4 xj = nj.x, vj = nj.v;
@@ -125,6 +126,7 @@ __global__ void k_solver_begin(Dev d, int which) {
 // divergence_solver_iteration_kernel, :1721-1760 (Jacobi: reads rho-dot and
 // alpha, writes v).  Reactions on dynamic solids are not formed: the second
 // clear_accelerations (:1552) erases them.
+template <int B>
 __global__ void k_div_update(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -135,7 +137,7 @@ __global__ void k_div_update(Dev d) {
   const float ki = d.src[i] * alpha_div(sc, d.alpha[i]);
   const int cnt = d.nnbr[i];
   float dvx = 0.f, dvy = 0.f, dvz = 0.f;
-  for_neighbors<8>(
+  for_neighbors<B>(
       d, i, cnt, [&](int j) { return PosSrc{d.pos[j], d.src[j], d.alpha[j]}; },
       [&](int, const PosSrc& nj) {
         const float4 xj = nj.x;
@@ -169,11 +171,12 @@ __global__ void k_div_update(Dev d) {
 }
 
 // sum_j V_j (v_i - v_j) . gradW_ij over all neighbors, and the count
+template <int B>
 __device__ __forceinline__ float velocity_divergence(const Dev& d, const SceneDev& sc, int i, float4 xi, int& cnt) {
   const float4 vi = d.vel[i];
   cnt = d.nnbr[i];
   float acc = 0.f;
-  for_neighbors<8>(
+  for_neighbors<B>(
       d, i, cnt, [&](int j) { return PosVel{d.pos[j], d.vel[j]}; },
       [&](int, const PosVel& nj) {
         const float4 xj = nj.x, vj = nj.v;
@@ -187,6 +190,7 @@ __device__ __forceinline__ float velocity_divergence(const Dev& d, const SceneDe
 
 // compute_density_change (:1610-1645) + compute_density_error(0) (:1669-1675)
 // and the loop test of divergence_solve (:1695-1703)
+template <int B>
 __global__ void k_div_source(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -196,7 +200,7 @@ __global__ void k_div_source(Dev d) {
   double v[1] = {0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
     int cnt;
-    float rd = velocity_divergence(d, sc, i, d.pos[i], cnt);
+    float rd = velocity_divergence<B>(d, sc, i, d.pos[i], cnt);
     rd = rd > 0.0f ? rd : 0.0f;
     if (cnt < 20) rd = 0.0f;
     d.src[i] = rd;
@@ -210,6 +214,7 @@ __global__ void k_div_source(Dev d) {
 // the implicit solver ran, then non_pressure_advect (:1843-1848).  The second
 // clear_accelerations (:1552) is folded in.  v is double-buffered: the sweep
 // reads vel and writes vel_alt.
+template <int B>
 __global__ void k_nonpressure(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -237,7 +242,7 @@ __global__ void k_nonpressure(Dev d) {
         float4 x, v;
         float rho;
       };
-      for_neighbors<8>(
+      for_neighbors<B>(
           d, i, cnt, [&](int j) { return NJ{d.pos[j], d.vel[j], d.rho[j]}; },
           [&](int, const NJ& nj) {
             const float4 xj = nj.x, vj = nj.v;
@@ -327,6 +332,7 @@ __global__ void k_nonpressure(Dev d) {
 
 // compute_density_adv (:1647-1667) + compute_density_error(rho0) and the loop
 // test of pressure_solve (:1778-1784)
+template <int B>
 __global__ void k_press_source(Dev d, int reduce) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -336,7 +342,7 @@ __global__ void k_press_source(Dev d, int reduce) {
   double v[1] = {0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
     int cnt;
-    const float delta = velocity_divergence(d, sc, i, d.pos[i], cnt);
+    const float delta = velocity_divergence<B>(d, sc, i, d.pos[i], cnt);
     float da = d.rho[i] / sc.rho0 + sc.dt * delta;
     da = da > 1.0f ? da : 1.0f;
     d.src[i] = da;
@@ -453,6 +459,7 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int whic
 // pressure_solve_iteration_kernel, :1799-1841.  Fluid lanes update v in place
 // (nothing in the sweep reads v); dynamic-solid lanes gather the reaction
 // -vel_change/dt of every fluid neighbor (:1835-1841) into their acceleration.
+template <int B>
 __global__ void k_press_update(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -466,7 +473,7 @@ __global__ void k_press_update(Dev d) {
   if (kind == KIND_FLUID) {
     const float ki = (d.src[i] - 1.0f) * alpha_press(sc, d.alpha[i]);
     float4 v = d.vel[i];
-    for_neighbors<8>(
+    for_neighbors<B>(
         d, i, cnt, [&](int j) { return PosSrc{d.pos[j], d.src[j], d.alpha[j]}; },
         [&](int, const PosSrc& nj) {
           const float4 xj = nj.x;
@@ -567,26 +574,26 @@ void launch_step_begin(const Dev& d, hipStream_t st) {
 void launch_clear_acc(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d);
 }
-void launch_density_alpha(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_density_alpha, grid_of(d, block), dim3(block), 0, st, d);
+void launch_density_alpha(const Dev& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_density_alpha, depth, grid_of(d, block), dim3(block), st, d);
 }
 void launch_solver_begin(const Dev& d, int which, hipStream_t st) {
   hipLaunchKernelGGL(k_solver_begin, dim3(1), dim3(256), 0, st, d, which);
 }
-void launch_div_update(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_div_update, grid_of(d, block), dim3(block), 0, st, d);
+void launch_div_update(const Dev& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_div_update, depth, grid_of(d, block), dim3(block), st, d);
 }
-void launch_div_source(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_div_source, grid_of(d, block), dim3(block), 0, st, d);
+void launch_div_source(const Dev& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_div_source, depth, grid_of(d, block), dim3(block), st, d);
 }
-void launch_nonpressure(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_nonpressure, grid_of(d, block), dim3(block), 0, st, d);
+void launch_nonpressure(const Dev& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_nonpressure, depth, grid_of(d, block), dim3(block), st, d);
 }
-void launch_press_source(const Dev& d, int block, int reduce, hipStream_t st) {
-  hipLaunchKernelGGL(k_press_source, grid_of(d, block), dim3(block), 0, st, d, reduce);
+void launch_press_source(const Dev& d, int block, int depth, int reduce, hipStream_t st) {
+  DEPTH_LAUNCH(k_press_source, depth, grid_of(d, block), dim3(block), st, d, reduce);
 }
-void launch_press_update(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_press_update, grid_of(d, block), dim3(block), 0, st, d);
+void launch_press_update(const Dev& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_press_update, depth, grid_of(d, block), dim3(block), st, d);
 }
 void launch_advect(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d);

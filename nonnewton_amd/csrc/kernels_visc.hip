@@ -36,6 +36,7 @@ __device__ __forceinline__ bool weiler_scene(const SceneDev& sc) {
 
 // NonNewton.calcStrainRate (:461-492, with the accumulation the reference
 // omits) + the viscosity model; mu for the implicit solver.
+template <int B>
 __global__ void k_strain(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
@@ -53,7 +54,7 @@ __global__ void k_strain(Dev d) {
     float S[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) S[q] = 0.0f;
-    for_neighbors<8>(
+    for_neighbors<B>(
         d, i, cnt, [&](int j) { return PosVel{d.pos[j], d.vel[j]}; },
         [&](int, const PosVel& nj) {
           const float4 xj = nj.x, vj = nj.v;
@@ -228,7 +229,11 @@ __global__ void k_cg_guess(Dev d) {
 
 // w = A u (u gathered from the neighbors); partial u.w into partial array
 // 3 (start: after cg_init, every Weiler scene) or 2 (an iteration, active
-// scenes).
+// scenes).  D list positions of a lane per batch: their list words and pair
+// scalars, then their neighbors' positions and u, are loaded before any of
+// them is used (a shorter chain of dependent loads); the terms are added in
+// list order whatever D is.
+template <int D>
 __global__ void k_cg_matvec(Dev d, int start) {
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
@@ -244,30 +249,30 @@ __global__ void k_cg_matvec(Dev d, int start) {
     ui = d.wu[i];
     const int cnt = d.nnbr[i];
     const long P = d.P;
-    int k = q;
-    // two list positions of this lane per batch: list words, pair scalars,
-    // then the neighbors' positions and u are loaded together
-    for (; k < cnt; k += 8) {
-      const bool two = k + 4 < cnt;
-      const int j0 = d.nbr[k * P + i];
-      const int j1 = two ? d.nbr[(k + 4) * P + i] : i;
-      const float s0 = d.spair[k * P + i];
-      const float s1 = two ? d.spair[(k + 4) * P + i] : 0.0f;
-      const float4 x0 = d.pos[j0], x1 = d.pos[j1];
-      const float4 u0 = d.wu[j0], u1 = d.wu[j1];
-      if (x0.w > 0.0f) {
-        const float rx = xi.x - x0.x, ry = xi.y - x0.y, rz = xi.z - x0.z;
-        const float t = s0 * ((ui.x - u0.x) * rx + (ui.y - u0.y) * ry + (ui.z - u0.z) * rz);
-        ax += t * rx;
-        ay += t * ry;
-        az += t * rz;
+    for (int k = q; k < cnt; k += 4 * D) {
+      int j[D];
+      float sp[D];
+      float4 xj[D], uj[D];
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        const bool in = k + 4 * e < cnt;
+        j[e] = in ? d.nbr[(k + 4 * e) * P + i] : i;
+        sp[e] = in ? d.spair[(k + 4 * e) * P + i] : 0.0f;
       }
-      if (two && x1.w > 0.0f) {
-        const float rx = xi.x - x1.x, ry = xi.y - x1.y, rz = xi.z - x1.z;
-        const float t = s1 * ((ui.x - u1.x) * rx + (ui.y - u1.y) * ry + (ui.z - u1.z) * rz);
-        ax += t * rx;
-        ay += t * ry;
-        az += t * rz;
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        xj[e] = d.pos[j[e]];
+        uj[e] = d.wu[j[e]];
+      }
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        if (k + 4 * e < cnt && xj[e].w > 0.0f) {
+          const float rx = xi.x - xj[e].x, ry = xi.y - xj[e].y, rz = xi.z - xj[e].z;
+          const float t = sp[e] * ((ui.x - uj[e].x) * rx + (ui.y - uj[e].y) * ry + (ui.z - uj[e].z) * rz);
+          ax += t * rx;
+          ay += t * ry;
+          az += t * rz;
+        }
       }
     }
   }
@@ -341,8 +346,8 @@ static inline dim3 grid4_of(const Dev& d, int block) {
   return dim3((4 * d.P + qb - 1) / qb);
 }
 
-void launch_strain(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_strain, grid_of(d, block), dim3(block), 0, st, d);
+void launch_strain(const Dev& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_strain, depth, grid_of(d, block), dim3(block), st, d);
 }
 void launch_cg_init(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d);
@@ -350,8 +355,14 @@ void launch_cg_init(const Dev& d, int block, hipStream_t st) {
 void launch_cg_guess(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d);
 }
-void launch_cg_matvec(const Dev& d, int block, int start, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_matvec, grid4_of(d, block), dim3(quad_block(block)), 0, st, d, start);
+void launch_cg_matvec(const Dev& d, int block, int depth, int start, hipStream_t st) {
+  const dim3 g = grid4_of(d, block), b = dim3(quad_block(block));
+  switch (depth) {
+    case 1: hipLaunchKernelGGL(k_cg_matvec<1>, g, b, 0, st, d, start); break;
+    case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d, start); break;
+    case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d, start); break;
+    default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d, start); break;
+  }
 }
 void launch_cg_update(const Dev& d, int block, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d);

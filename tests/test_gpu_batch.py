@@ -47,9 +47,10 @@ def test_pool_equals_alone_and_churn():
 
 
 def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
-    """Every kernel at 64-, 1024- and autotuned workgroups: the same bits as the
+    """Every kernel at 64- and 1024-thread workgroups with the largest and the
+    smallest gather depths, and with autotuned shapes: the same bits as the
     default shapes (the partial sums are grouped per wavefront, not per
-    workgroup)."""
+    workgroup, and every sum keeps its order)."""
     from nonnewton_amd import autotune
     monkeypatch.setenv("NONNEWTON_AMD_TUNE_CACHE", str(tmp_path / "shapes.json"))
     batch = scenes.mixed_batch(count=3, lo=2000, hi=5000)
@@ -61,14 +62,20 @@ def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
         pool = L.Pool(8 << 30)
         slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in batch]
         if mode == "tuned":
-            shapes = autotune.tune_pool(pool, len(batch), sum(s.n for s in batch), ticks=1,
-                                        candidates=(64, 256, 1024))
-            assert shapes and all(b in (64, 256, 1024) for b in shapes.values())
-            steps_done = 6  # the tuner stepped the pool 2 x 3 x 1 times
+            shapes = autotune.tune_pool(pool, slots, [s.n_fluid for s in batch], rounds=1,
+                                        candidates=(64, 1024), depth_levels=(1, 16), verify_rounds=1)
+            assert all(sh["block"] in (0, 64, 1024) and sh["depth"] in (0, 1, 16) for sh in shapes.values())
+            steps_done = 6  # 1 round x (2 sizes + 2 depths) + 2 verification ticks
+            if not shapes:  # the defaults won: run the same check with a forced choice
+                autotune.apply(pool, {"cg_matvec": {"block": 64, "depth": 8},
+                                      "density_alpha": {"block": 1024, "depth": 16},
+                                      "press_source": {"block": 128, "depth": 4}})
         else:
             for k in L.kernel_names():
                 if k != "scan":
                     pool.set_kernel_block(k, mode)
+            for k in autotune.DEPTHS:
+                pool.set_kernel_depth(k, 16 if mode == 64 else 1)
             steps_done = 0
         for _ in range(6 - steps_done):
             pool.tick()

@@ -210,21 +210,25 @@ int32_t nn_set_kernel_block(nn_ctx *ctx, int32_t kernel, int32_t block);
 int32_t nn_pool_set_kernel_depth(nn_pool *pool, int32_t kernel, int32_t depth);
 int32_t nn_set_kernel_depth(nn_ctx *ctx, int32_t kernel, int32_t depth);
 
-/* Per-kernel timing with HIP events recorded around every launch on the stream
- * the kernel runs on (no counterpart in the reference).  enable != 0 turns it
- * on for every kernel and clears the totals; the _mask forms time only the
- * kernel ids whose bit is set (0 = off), so the other launches run without
- * events between them.  nn_*_kernel_times copies the accumulated milliseconds
- * and launch counts of kernel ids 0..n-1 (names from nn_kernel_name,
- * NN_KERNEL_COUNT ids). */
+/* Per-kernel timing with HIP events recorded around launches on the stream the
+ * kernel runs on (no counterpart in the reference).  enable != 0 turns it on
+ * for every launch of every kernel and clears the totals.  The _mask forms
+ * time only the kernel ids whose bit is set (0 = off), and of those only every
+ * stride-th launch, so that the other launches run without events between
+ * them.  nn_*_kernel_times copies the accumulated milliseconds and the number
+ * of timed launches of kernel ids 0..n-1 (names from nn_kernel_name,
+ * NN_KERNEL_COUNT ids); nn_*_kernel_launches the number of launches made while
+ * timing was on, timed or not. */
 #define NN_KERNEL_COUNT 21
 const char *nn_kernel_name(int32_t id);
 int32_t nn_profile(nn_ctx *ctx, int32_t enable);
-int32_t nn_profile_mask(nn_ctx *ctx, uint32_t mask);
+int32_t nn_profile_mask(nn_ctx *ctx, uint32_t mask, int32_t stride);
 int32_t nn_kernel_times(nn_ctx *ctx, double *ms, int64_t *launches, int32_t n);
+int32_t nn_kernel_launches(nn_ctx *ctx, int64_t *seen, int32_t n);
 int32_t nn_pool_profile(nn_pool *pool, int32_t enable);
-int32_t nn_pool_profile_mask(nn_pool *pool, uint32_t mask);
+int32_t nn_pool_profile_mask(nn_pool *pool, uint32_t mask, int32_t stride);
 int32_t nn_pool_kernel_times(nn_pool *pool, double *ms, int64_t *launches, int32_t n);
+int32_t nn_pool_kernel_launches(nn_pool *pool, int64_t *seen, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -150,10 +150,12 @@ def load():
         "nn_set_kernel_depth": (i32, [vp, i32, i32]),
         "nn_pool_set_kernel_depth": (i32, [vp, i32, i32]),
         "nn_pool_set_kernel_block": (i32, [vp, i32, i32]),
-        "nn_profile_mask": (i32, [vp, ctypes.c_uint32]),
+        "nn_profile_mask": (i32, [vp, ctypes.c_uint32, i32]),
+        "nn_kernel_launches": (i32, [vp, P(i64), i32]),
         "nn_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
         "nn_pool_profile": (i32, [vp, i32]),
-        "nn_pool_profile_mask": (i32, [vp, ctypes.c_uint32]),
+        "nn_pool_profile_mask": (i32, [vp, ctypes.c_uint32, i32]),
+        "nn_pool_kernel_launches": (i32, [vp, P(i64), i32]),
         "nn_pool_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
     }
     for name, (res, args) in sig.items():
@@ -191,6 +193,12 @@ def kernel_mask(kernels) -> int:
 def kernel_names():
     lib = load()
     return [lib.nn_kernel_name(k).decode() for k in range(KERNEL_COUNT)]
+
+
+def _seen(fn, h):
+    n = (ctypes.c_int64 * KERNEL_COUNT)()
+    check(fn(h, n, KERNEL_COUNT), "kernel_launches")
+    return {name: int(n[k]) for k, name in enumerate(kernel_names())}
 
 
 def _times(fn, h):
@@ -298,12 +306,18 @@ class Context:
     def set_kernel_depth(self, kernel: str, depth: int):
         check(self.lib.nn_set_kernel_depth(self.h, kernel_names().index(kernel), int(depth)), "nn_set_kernel_depth")
 
-    def profile(self, enable: bool = True, kernels=None):
-        """Time every kernel (kernels None) or only the named ones."""
-        if kernels is None:
+    def profile(self, enable: bool = True, kernels=None, stride: int = 1):
+        """Time every launch of every kernel (kernels None), or every stride-th
+        launch of the named ones."""
+        if kernels is None and stride == 1:
             check(self.lib.nn_profile(self.h, int(enable)), "nn_profile")
         else:
-            check(self.lib.nn_profile_mask(self.h, kernel_mask(kernels) if enable else 0), "nn_profile_mask")
+            m = (kernel_mask(kernels) if kernels is not None else 0xFFFFFFFF) if enable else 0
+            check(self.lib.nn_profile_mask(self.h, m, int(stride)), "nn_profile_mask")
+
+    def kernel_launches(self):
+        """{kernel name: launches made while timing was on (timed or not)}."""
+        return _seen(self.lib.nn_kernel_launches, self.h)
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since profiling was enabled."""
@@ -399,13 +413,18 @@ class Pool:
         check(self.lib.nn_pool_set_kernel_depth(self.h, kernel_names().index(kernel), int(depth)),
               "nn_pool_set_kernel_depth")
 
-    def profile(self, enable: bool = True, kernels=None):
-        """Time every kernel (kernels None) or only the named ones."""
-        if kernels is None:
+    def profile(self, enable: bool = True, kernels=None, stride: int = 1):
+        """Time every launch of every kernel (kernels None), or every stride-th
+        launch of the named ones."""
+        if kernels is None and stride == 1:
             check(self.lib.nn_pool_profile(self.h, int(enable)), "nn_pool_profile")
         else:
-            check(self.lib.nn_pool_profile_mask(self.h, kernel_mask(kernels) if enable else 0),
-                  "nn_pool_profile_mask")
+            m = (kernel_mask(kernels) if kernels is not None else 0xFFFFFFFF) if enable else 0
+            check(self.lib.nn_pool_profile_mask(self.h, m, int(stride)), "nn_pool_profile_mask")
+
+    def kernel_launches(self):
+        """{kernel name: launches made while timing was on (timed or not)}."""
+        return _seen(self.lib.nn_pool_kernel_launches, self.h)
 
     def kernel_times(self):
         return _times(self.lib.nn_pool_kernel_times, self.h)

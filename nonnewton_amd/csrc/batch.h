@@ -52,7 +52,9 @@ const char* kernel_name(int id);
 struct KernelTimer {
   bool on = false;
   unsigned mask = ~0u;  // kernel ids timed while on
+  int stride = 1;       // time every stride-th launch of each timed kernel
   bool open = false;    // a start event was recorded for the launch in flight
+  long long seen[K_COUNT] = {};  // launches of each kernel while on (timed or not)
   double total_ms[K_COUNT] = {};
   long long count[K_COUNT] = {};
   std::vector<hipEvent_t> events;
@@ -78,7 +80,21 @@ class Batch {
   // Lay the current scenes out and allocate device memory.  When `from` is
   // given, every scene already present in `from` (same SceneHost object) has
   // its persistent state copied from there; other scenes must be uploaded.
-  int layout(const std::vector<const Batch*>& from);
+  // The arrays get `slack` times the room the scenes need (and room for
+  // `scene_capacity` scenes), so that append_in_place can add scenes later.
+  int layout(const std::vector<const Batch*>& from, double slack = 1.0, int scene_capacity = 0);
+  // Change membership without moving any scene: `members` lists the scenes
+  // kept, in their present order, followed by new scenes, which are placed
+  // after the last used slot, cell and stress slot and get their state from
+  // `from`.  Retired scenes leave unused ranges whose waves map to no scene.
+  // Returns LAYOUT_NEEDED (nothing changed) when the order or the room does
+  // not allow it.
+  static constexpr int LAYOUT_NEEDED = 2;
+  int append_in_place(const std::vector<std::shared_ptr<SceneHost>>& members, const std::vector<const Batch*>& from);
+  // When set, arrays replaced by layout() are not freed but handed out by
+  // take_deferred(), to be freed off the thread that steps the batch.
+  bool defer_free_ = false;
+  void take_deferred(std::vector<void*>& out);
   int upload(int k, const float* x, const float* v, const int* kind);
   int init_scenes(const std::vector<int>& which);  // search, rest state, boundary volumes
   int search();
@@ -105,7 +121,10 @@ class Batch {
   };
   std::vector<Alloc> allocs_;
   long long bytes_ = 0;
-  int C_total_ = 0;
+  int C_total_ = 0;  // cells in use
+  int C_cap_ = 0;    // cells allocated
+  int scene_cap_ = 0;
+  std::vector<void*> deferred_;
   int scan_blocks_ = 0;
   SceneState* h_state_ = nullptr;  // pinned
   int* h_active_ = nullptr;        // pinned
@@ -121,6 +140,9 @@ class Batch {
   int build_stress_set(const std::vector<int>& which);
   int gather_state();
   void swap_sort_buffers();
+  void derive_members();
+  int upload_tables(const Dev& nd);
+  int copy_scene_state(const Dev& nd, const SceneHost& s, const Batch* from, int ob, int osb, int ons);
 };
 
 }  // namespace nn

@@ -214,40 +214,23 @@ void Batch::swap_sort_buffers() {
   std::swap(d.vdiff, d.ws);  // k_place moved the guesses into ws (free outside the solve)
 }
 
-int Batch::layout(const std::vector<const Batch*>& from) {
-  struct Src {
-    const Batch* b;
-    int pb, sb, ns;  // offsets and stress-set size in the source batch
-  };
-  std::vector<Src> src(scenes.size(), Src{nullptr, 0, 0, 0});
-  for (size_t k = 0; k < scenes.size(); k++) {
-    for (const Batch* b : from) {
-      if (!b) continue;
-      for (auto& t : b->scenes)
-        if (t.get() == scenes[k].get()) src[k] = Src{b, scenes[k]->p_begin, scenes[k]->s_begin, scenes[k]->dev.ns};
-    }
-  }
-  // offsets
-  int P = 0, C = 0, NS = 0;
+// Per-scene fields that depend on the scenes' order and offsets (p_begin,
+// cell_begin and s_begin must be set): the slot shift used by the sort, the
+// wave range, and the batch-wide feature flags.
+void Batch::derive_members() {
   long base = 0;
   any_fluid_ = any_elastic_ = any_weiler_ = any_strain_ = any_div_ = any_warm_cg_ = false;
   max_iter_[0] = max_iter_[1] = max_iter_[2] = 1;
   for (auto& sp : scenes) {
     SceneHost& s = *sp;
-    s.p_begin = P;
-    s.dev.p_begin = P;
-    s.dev.shift = (int)(P - base);
-    s.dev.wave_begin = P / 64;
-    base += s.n;
-    P += round_up64(s.n);
-    s.dev.wave_end = P / 64;
-    s.cell_begin = C;
-    s.dev.cell_begin = C;
-    C += (int)s.ncells + 1;
-    s.s_begin = NS;
-    s.dev.s_begin = NS;
+    s.dev.p_begin = s.p_begin;
+    s.dev.shift = (int)(s.p_begin - base);  // particles of the scenes before this one, in cell order
+    s.dev.wave_begin = s.p_begin / 64;
+    s.dev.wave_end = (s.p_begin + round_up64(s.n)) / 64;
+    s.dev.cell_begin = s.cell_begin;
+    s.dev.s_begin = s.s_begin;
     s.dev.ns = s.ns;
-    NS += round_up64(s.ns);
+    base += s.n;
     const int f = s.dev.flags;
     if (f & SF_HAS_FLUID) {
       any_fluid_ = true;
@@ -261,87 +244,12 @@ int Batch::layout(const std::vector<const Batch*>& from) {
     max_iter_[1] = std::max(max_iter_[1], s.dev.max_iter_p);
     max_iter_[2] = std::max(max_iter_[2], s.dev.max_iter_cg);
   }
-  // new arrays
-  Dev nd{};
-  std::vector<Alloc> old_allocs;
-  old_allocs.swap(allocs_);
-  long long old_bytes = bytes_;
-  bytes_ = 0;
-  nd.P = P;
-  nd.W = P / 64;
-  nd.C = C;
-  nd.NS = NS;
-  nd.SW = NS / 64;
-  nd.nscenes = (int)scenes.size();
-  C_total_ = C;
-  const int ncell_entries = C + 1;
-  scan_blocks_ = (ncell_entries + 2047) / 2048;
-  int rc = NN_OK;
-  auto A = [&](auto*& p, size_t count) {
-    if (rc == NN_OK) rc = alloc(p, count);
-  };
-  SceneDev* scenes_dev;
-  A(scenes_dev, scenes.size());
-  nd.scenes = scenes_dev;
-  A(nd.state, scenes.size());
-  int* ws;
-  A(ws, nd.W);
-  nd.wave_scene = ws;
-  int* sws;
-  A(sws, nd.SW);
-  nd.s_wave_scene = sws;
-  A(nd.pos, P);
-  A(nd.pos_alt, P);
-  A(nd.vel, P);
-  A(nd.vel_alt, P);
-  A(nd.info, P);
-  A(nd.info_alt, P);
-  A(nd.acc, P);
-  A(nd.rho, P);
-  A(nd.alpha, P);
-  A(nd.src, P);
-  A(nd.nbr, (size_t)P * MAX_NB);
-  A(nd.nnbr, P);
-  A(nd.key, P);
-  A(nd.rank, P);
-  A(nd.inv, P);
-  A(nd.bucket, P);
-  A(nd.cell_count, ncell_entries);
-  A(nd.cell_start, ncell_entries);
-  A(nd.scan_tmp, scan_blocks_);
-  A(nd.partials, 16 * (size_t)nd.W);
-  A(nd.n_active, 4);
-  A(nd.wx, P);
-  A(nd.wr, P);
-  A(nd.wu, P);
-  A(nd.ww, P);
-  A(nd.wp, P);
-  A(nd.ws, P);
-  A(nd.wd0, P);
-  A(nd.wd1, P);
-  A(nd.spair, (size_t)P * MAX_NB);
-  A(nd.vdiff, P);
-  A(nd.mu, P);
-  A(nd.gam, P);
-  A(nd.eta, P);
-  A(nd.x0, P);
-  A(nd.s_of, P);
-  A(nd.s_pid, NS);
-  A(nd.rest, (size_t)NS * MAX_REST);
-  A(nd.rest_cnt, NS);
-  A(nd.rs, 4 * (size_t)NS);
-  if (rc != NN_OK) {
-    release();
-    allocs_.swap(old_allocs);
-    bytes_ = old_bytes;
-    return rc;
-  }
-  if (!h_state_) {
-    HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * 4096, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
-  }
-  // tables
-  std::vector<int> hw(std::max(nd.W, 1)), hsw(std::max(nd.SW, 1));
+}
+
+// Scene table and the wave -> scene maps of `nd` for the current scenes
+// (waves of no scene map to -1).
+int Batch::upload_tables(const Dev& nd) {
+  std::vector<int> hw(std::max(nd.W, 1), -1), hsw(std::max(nd.SW, 1), -1);
   std::vector<SceneDev> hs(scenes.size());
   for (size_t k = 0; k < scenes.size(); k++) {
     SceneHost& s = *scenes[k];
@@ -349,64 +257,265 @@ int Batch::layout(const std::vector<const Batch*>& from) {
     for (int w = s.s_begin / 64; w < (s.s_begin + round_up64(s.ns)) / 64; w++) hsw[w] = (int)k;
     hs[k] = s.dev;
   }
-  if (!scenes.empty()) HIPCHK(hipMemcpyAsync((void*)nd.scenes, hs.data(), sizeof(SceneDev) * hs.size(), hipMemcpyHostToDevice, stream));
+  if (!scenes.empty())
+    HIPCHK(hipMemcpyAsync((void*)nd.scenes, hs.data(), sizeof(SceneDev) * hs.size(), hipMemcpyHostToDevice, stream));
   if (nd.W) HIPCHK(hipMemcpyAsync((void*)nd.wave_scene, hw.data(), sizeof(int) * nd.W, hipMemcpyHostToDevice, stream));
   if (nd.SW) HIPCHK(hipMemcpyAsync((void*)nd.s_wave_scene, hsw.data(), sizeof(int) * nd.SW, hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemsetAsync(nd.state, 0, sizeof(SceneState) * std::max<size_t>(scenes.size(), 1), stream));
-  launch_fill_i32(nd.info, P, -1, stream);
-  launch_fill_i32(nd.info_alt, P, -1, stream);
-  launch_fill_i32(nd.s_of, P, -1, stream);
-  launch_fill_i32(nd.s_pid, NS, -1, stream);
-  launch_fill_i32(nd.rest_cnt, NS, 0, stream);
-  launch_fill_i32(nd.nnbr, P, 0, stream);
-  launch_fill_f4(nd.pos, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
-  launch_fill_f4(nd.pos_alt, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
-  launch_fill_f4(nd.vel, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
-  launch_fill_f4(nd.vel_alt, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
-  launch_fill_f4(nd.x0, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
-  launch_fill_f4(nd.vdiff, P, make_float4(0.f, 0.f, 0.f, 0.f), stream);
-  HIPCHK(hipMemsetAsync(nd.rho, 0, sizeof(float) * std::max(P, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.alpha, 0, sizeof(float) * std::max(P, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.src, 0, sizeof(float) * std::max(P, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.gam, 0, sizeof(float) * std::max(P, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.eta, 0, sizeof(float) * std::max(P, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.acc, 0, sizeof(float4) * std::max(P, 1), stream));
-  // copy persistent state of relocated scenes
+  // the host vectors must outlive the copies
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+// Persistent state of scene s (at its offsets in nd) from batch `from`, where
+// it sits at slot ob and stress-set slot osb with a stress set of ons members.
+int Batch::copy_scene_state(const Dev& nd, const SceneHost& s, const Batch* from, int ob, int osb, int ons) {
+  const Dev& od = from->d;
+  const int nb = s.p_begin, n = s.n;
+  auto cp = [&](void* dst, const void* sp, size_t bytes) -> hipError_t {
+    return bytes ? hipMemcpyAsync(dst, sp, bytes, hipMemcpyDeviceToDevice, stream) : hipSuccess;
+  };
+  HIPCHK(cp(nd.pos + nb, od.pos + ob, sizeof(float4) * n));
+  HIPCHK(cp(nd.vel + nb, od.vel + ob, sizeof(float4) * n));
+  HIPCHK(cp(nd.info + nb, od.info + ob, sizeof(int) * n));
+  HIPCHK(cp(nd.inv + nb, od.inv + ob, sizeof(int) * n));
+  HIPCHK(cp(nd.x0 + nb, od.x0 + ob, sizeof(float4) * n));
+  HIPCHK(cp(nd.s_of + nb, od.s_of + ob, sizeof(int) * n));
+  HIPCHK(cp(nd.rho + nb, od.rho + ob, sizeof(float) * n));
+  HIPCHK(cp(nd.alpha + nb, od.alpha + ob, sizeof(float) * n));
+  HIPCHK(cp(nd.src + nb, od.src + ob, sizeof(float) * n));
+  HIPCHK(cp(nd.acc + nb, od.acc + ob, sizeof(float4) * n));
+  HIPCHK(cp(nd.gam + nb, od.gam + ob, sizeof(float) * n));
+  HIPCHK(cp(nd.eta + nb, od.eta + ob, sizeof(float) * n));
+  HIPCHK(cp(nd.vdiff + nb, od.vdiff + ob, sizeof(float4) * n));
+  if (s.ns > 0 && ons == s.ns) {  // a resized stress set is uploaded afresh
+    const int ns_ = s.s_begin;
+    HIPCHK(cp(nd.s_pid + ns_, od.s_pid + osb, sizeof(int) * s.ns));
+    HIPCHK(cp(nd.rest_cnt + ns_, od.rest_cnt + osb, sizeof(int) * s.ns));
+    HIPCHK(cp(nd.rs + 4L * ns_, od.rs + 4L * osb, sizeof(float4) * 4 * s.ns));
+    HIPCHK(hipMemcpy2DAsync(nd.rest + ns_, sizeof(int) * nd.NS, od.rest + osb, sizeof(int) * od.NS,
+                            sizeof(int) * s.ns, MAX_REST, hipMemcpyDeviceToDevice, stream));
+  }
+  return NN_OK;
+}
+
+int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene_capacity) {
+  struct Src {
+    const Batch* b;
+    int pb, sb, ns;  // offsets and stress-set size in the source batch
+  };
+  std::vector<Src> src(scenes.size(), Src{nullptr, 0, 0, 0});
   for (size_t k = 0; k < scenes.size(); k++) {
-    if (!src[k].b) continue;
-    const SceneHost& s = *scenes[k];
-    const Dev& od = src[k].b->d;
-    const int ob = src[k].pb, nb = s.p_begin, n = s.n;
-    auto cp = [&](void* dst, const void* sp, size_t bytes) -> hipError_t {
-      return bytes ? hipMemcpyAsync(dst, sp, bytes, hipMemcpyDeviceToDevice, stream) : hipSuccess;
-    };
-    HIPCHK(cp(nd.pos + nb, od.pos + ob, sizeof(float4) * n));
-    HIPCHK(cp(nd.vel + nb, od.vel + ob, sizeof(float4) * n));
-    HIPCHK(cp(nd.info + nb, od.info + ob, sizeof(int) * n));
-    HIPCHK(cp(nd.inv + nb, od.inv + ob, sizeof(int) * n));
-    HIPCHK(cp(nd.x0 + nb, od.x0 + ob, sizeof(float4) * n));
-    HIPCHK(cp(nd.s_of + nb, od.s_of + ob, sizeof(int) * n));
-    HIPCHK(cp(nd.rho + nb, od.rho + ob, sizeof(float) * n));
-    HIPCHK(cp(nd.alpha + nb, od.alpha + ob, sizeof(float) * n));
-    HIPCHK(cp(nd.src + nb, od.src + ob, sizeof(float) * n));
-    HIPCHK(cp(nd.acc + nb, od.acc + ob, sizeof(float4) * n));
-    HIPCHK(cp(nd.gam + nb, od.gam + ob, sizeof(float) * n));
-    HIPCHK(cp(nd.eta + nb, od.eta + ob, sizeof(float) * n));
-    HIPCHK(cp(nd.vdiff + nb, od.vdiff + ob, sizeof(float4) * n));
-    if (s.ns > 0 && src[k].ns == s.ns) {  // a resized stress set is uploaded afresh
-      const int os = src[k].sb, ns_ = s.s_begin;
-      HIPCHK(cp(nd.s_pid + ns_, od.s_pid + os, sizeof(int) * s.ns));
-      HIPCHK(cp(nd.rest_cnt + ns_, od.rest_cnt + os, sizeof(int) * s.ns));
-      HIPCHK(cp(nd.rs + 4L * ns_, od.rs + 4L * os, sizeof(float4) * 4 * s.ns));
-      HIPCHK(hipMemcpy2DAsync(nd.rest + ns_, sizeof(int) * nd.NS, od.rest + os, sizeof(int) * od.NS,
-                              sizeof(int) * s.ns, MAX_REST, hipMemcpyDeviceToDevice, stream));
+    for (const Batch* b : from) {
+      if (!b) continue;
+      for (auto& t : b->scenes)
+        if (t.get() == scenes[k].get()) src[k] = Src{b, scenes[k]->p_begin, scenes[k]->s_begin, scenes[k]->dev.ns};
     }
   }
+  // offsets, packed in member order (committed once the arrays exist)
+  int P = 0, C = 0, NS = 0;
+  std::vector<int> pb(scenes.size()), cb(scenes.size()), sb(scenes.size());
+  for (size_t k = 0; k < scenes.size(); k++) {
+    const SceneHost& s = *scenes[k];
+    pb[k] = P;
+    P += round_up64(s.n);
+    cb[k] = C;
+    C += (int)s.ncells + 1;
+    sb[k] = NS;
+    NS += round_up64(s.ns);
+  }
+  // capacities: the extents, grown by `slack` so that scenes can be appended
+  // later without moving the others
+  slack = slack < 1.0 ? 1.0 : slack;
+  const int Pc = round_up64((int)std::ceil(P * slack));
+  const int Cc = (int)std::ceil(C * slack);
+  const int NSc = round_up64((int)std::ceil(NS * slack));
+  const int Sc = std::max<int>((int)scenes.size(), scene_capacity);
+  // new arrays
+  Dev nd{};
+  std::vector<Alloc> old_allocs;
+  old_allocs.swap(allocs_);
+  long long old_bytes = bytes_;
+  bytes_ = 0;
+  nd.P = Pc;
+  nd.W = Pc / 64;
+  nd.Pn = P;
+  nd.C = C;
+  nd.NS = NSc;
+  nd.SW = NSc / 64;
+  nd.NSn = NS;
+  nd.nscenes = (int)scenes.size();
+  const int ncell_entries = Cc + 1;
+  const int nscan = (ncell_entries + 2047) / 2048;
+  int rc = NN_OK;
+  auto A = [&](auto*& p, size_t count) {
+    if (rc == NN_OK) rc = alloc(p, count);
+  };
+  SceneDev* scenes_dev;
+  A(scenes_dev, Sc);
+  nd.scenes = scenes_dev;
+  A(nd.state, Sc);
+  int* ws;
+  A(ws, nd.W);
+  nd.wave_scene = ws;
+  int* sws;
+  A(sws, nd.SW);
+  nd.s_wave_scene = sws;
+  A(nd.pos, Pc);
+  A(nd.pos_alt, Pc);
+  A(nd.vel, Pc);
+  A(nd.vel_alt, Pc);
+  A(nd.info, Pc);
+  A(nd.info_alt, Pc);
+  A(nd.acc, Pc);
+  A(nd.rho, Pc);
+  A(nd.alpha, Pc);
+  A(nd.src, Pc);
+  A(nd.nbr, (size_t)Pc * MAX_NB);
+  A(nd.nnbr, Pc);
+  A(nd.key, Pc);
+  A(nd.rank, Pc);
+  A(nd.inv, Pc);
+  A(nd.bucket, Pc);
+  A(nd.cell_count, ncell_entries);
+  A(nd.cell_start, ncell_entries);
+  A(nd.scan_tmp, nscan);
+  A(nd.partials, 16 * (size_t)nd.W);
+  A(nd.n_active, 4);
+  A(nd.wx, Pc);
+  A(nd.wr, Pc);
+  A(nd.wu, Pc);
+  A(nd.ww, Pc);
+  A(nd.wp, Pc);
+  A(nd.ws, Pc);
+  A(nd.wd0, Pc);
+  A(nd.wd1, Pc);
+  A(nd.spair, (size_t)Pc * MAX_NB);
+  A(nd.vdiff, Pc);
+  A(nd.mu, Pc);
+  A(nd.gam, Pc);
+  A(nd.eta, Pc);
+  A(nd.x0, Pc);
+  A(nd.s_of, Pc);
+  A(nd.s_pid, NSc);
+  A(nd.rest, (size_t)NSc * MAX_REST);
+  A(nd.rest_cnt, NSc);
+  A(nd.rs, 4 * (size_t)NSc);
+  if (rc != NN_OK) {
+    release();
+    allocs_.swap(old_allocs);
+    bytes_ = old_bytes;
+    return rc;
+  }
+  for (size_t k = 0; k < scenes.size(); k++) {
+    scenes[k]->p_begin = pb[k];
+    scenes[k]->cell_begin = cb[k];
+    scenes[k]->s_begin = sb[k];
+  }
+  derive_members();
+  C_total_ = C;
+  C_cap_ = Cc;
+  scene_cap_ = Sc;
+  scan_blocks_ = nscan;
+  if (!h_state_) {
+    HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * 4096, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
+  }
+  CHK(upload_tables(nd));
+  HIPCHK(hipMemsetAsync(nd.state, 0, sizeof(SceneState) * std::max<size_t>(Sc, 1), stream));
+  launch_fill_i32(nd.info, Pc, -1, stream);
+  launch_fill_i32(nd.info_alt, Pc, -1, stream);
+  launch_fill_i32(nd.s_of, Pc, -1, stream);
+  launch_fill_i32(nd.s_pid, NSc, -1, stream);
+  launch_fill_i32(nd.rest_cnt, NSc, 0, stream);
+  launch_fill_i32(nd.nnbr, Pc, 0, stream);
+  launch_fill_f4(nd.pos, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.pos_alt, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.vel, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.vel_alt, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.x0, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  launch_fill_f4(nd.vdiff, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
+  HIPCHK(hipMemsetAsync(nd.rho, 0, sizeof(float) * std::max(Pc, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.alpha, 0, sizeof(float) * std::max(Pc, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.src, 0, sizeof(float) * std::max(Pc, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.gam, 0, sizeof(float) * std::max(Pc, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.eta, 0, sizeof(float) * std::max(Pc, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.acc, 0, sizeof(float4) * std::max(Pc, 1), stream));
+  // copy persistent state of relocated scenes
+  for (size_t k = 0; k < scenes.size(); k++)
+    if (src[k].b) CHK(copy_scene_state(nd, *scenes[k], src[k].b, src[k].pb, src[k].sb, src[k].ns));
   HIPCHK(hipStreamSynchronize(stream));
-  // free the old arrays of this batch
-  for (auto& a : old_allocs) (void)hipFree(a.p);
+  // the old arrays are freed by the caller's choice: now, or later (off the
+  // thread that steps the batch; hipFree waits for the device)
+  if (defer_free_) {
+    for (auto& al : old_allocs) deferred_.push_back(al.p);
+  } else {
+    for (auto& al : old_allocs) (void)hipFree(al.p);
+  }
   d = nd;
   return NN_OK;
+}
+
+int Batch::append_in_place(const std::vector<std::shared_ptr<SceneHost>>& members, const std::vector<const Batch*>& from) {
+  // the scenes kept must come first, in their present order
+  std::vector<SceneHost*> cur;
+  for (auto& sp : scenes) cur.push_back(sp.get());
+  size_t nk = 0, pos = 0;
+  for (; nk < members.size(); nk++) {
+    auto it = std::find(cur.begin() + pos, cur.end(), members[nk].get());
+    if (it == cur.end()) break;
+    pos = (size_t)(it - cur.begin()) + 1;
+  }
+  for (size_t k = nk; k < members.size(); k++)
+    if (std::find(cur.begin(), cur.end(), members[k].get()) != cur.end()) return LAYOUT_NEEDED;
+  if ((int)members.size() > scene_cap_) return LAYOUT_NEEDED;
+  // offsets of the new scenes, after the last slot, cell and stress slot used
+  int P = d.Pn, C = C_total_, NS = d.NSn;
+  std::vector<std::pair<int, int>> place(members.size(), {0, 0});  // (p_begin, s_begin) of new scenes
+  std::vector<int> cells(members.size(), 0);
+  for (size_t k = nk; k < members.size(); k++) {
+    const SceneHost& s = *members[k];
+    place[k] = {P, NS};
+    cells[k] = C;
+    P += round_up64(s.n);
+    C += (int)s.ncells + 1;
+    NS += round_up64(s.ns);
+  }
+  if (P > d.P || C > C_cap_ || NS > d.NS) return LAYOUT_NEEDED;
+  // sources of the new scenes
+  struct Src {
+    const Batch* b;
+    int pb, sb, ns;
+  };
+  std::vector<Src> src(members.size(), Src{nullptr, 0, 0, 0});
+  for (size_t k = nk; k < members.size(); k++)
+    for (const Batch* b : from) {
+      if (!b || b == this) continue;
+      for (auto& t : b->scenes)
+        if (t.get() == members[k].get()) src[k] = Src{b, t->p_begin, t->s_begin, t->dev.ns};
+    }
+  for (size_t k = nk; k < members.size(); k++) {
+    SceneHost& s = *members[k];
+    s.p_begin = place[k].first;
+    s.s_begin = place[k].second;
+    s.cell_begin = cells[k];
+  }
+  scenes = members;
+  derive_members();
+  d.Pn = P;
+  d.NSn = NS;
+  d.nscenes = (int)scenes.size();
+  C_total_ = C;
+  d.C = C;
+  CHK(upload_tables(d));
+  for (size_t k = nk; k < members.size(); k++)
+    if (src[k].b) CHK(copy_scene_state(d, *members[k], src[k].b, src[k].pb, src[k].sb, src[k].ns));
+  HIPCHK(hipStreamSynchronize(stream));
+  return NN_OK;
+}
+
+void Batch::take_deferred(std::vector<void*>& out) {
+  out.insert(out.end(), deferred_.begin(), deferred_.end());
+  deferred_.clear();
 }
 
 int Batch::refresh_scene_table() {
@@ -465,6 +574,8 @@ KernelTimer::~KernelTimer() {
 int KernelTimer::begin(int id, hipStream_t st) {
   open = on && ((mask >> id) & 1u);
   if (!open) return NN_OK;
+  open = (seen[id]++ % stride) == 0;
+  if (!open) return NN_OK;
   while (events.size() < used + 2) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
@@ -499,6 +610,7 @@ void KernelTimer::reset() {
   for (int k = 0; k < K_COUNT; k++) {
     total_ms[k] = 0.0;
     count[k] = 0;
+    seen[k] = 0;
   }
 }
 

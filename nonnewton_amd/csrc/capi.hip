@@ -227,14 +227,21 @@ static int copy_times(const nn::KernelTimer& t, double* ms, int64_t* launches, i
   return NN_OK;
 }
 
+static int copy_seen(const nn::KernelTimer& t, int64_t* seen, int32_t n) {
+  if (!seen) return NN_E_ARG;
+  for (int k = 0; k < n && k < nn::K_COUNT; k++) seen[k] = t.seen[k];
+  return NN_OK;
+}
+
 const char* nn_kernel_name(int32_t id) { return nn::kernel_name(id); }
 
-int32_t nn_profile(nn_ctx* ctx, int32_t enable) { return nn_profile_mask(ctx, enable ? 0xFFFFFFFFu : 0u); }
+int32_t nn_profile(nn_ctx* ctx, int32_t enable) { return nn_profile_mask(ctx, enable ? 0xFFFFFFFFu : 0u, 1); }
 
-int32_t nn_profile_mask(nn_ctx* ctx, uint32_t mask) {
-  if (!ctx) return NN_E_ARG;
+int32_t nn_profile_mask(nn_ctx* ctx, uint32_t mask, int32_t stride) {
+  if (!ctx || stride < 1) return NN_E_ARG;
   ctx->batch->timer.on = mask != 0;
   ctx->batch->timer.mask = mask;
+  ctx->batch->timer.stride = stride;
   ctx->batch->timer.reset();
   return NN_OK;
 }
@@ -242,6 +249,11 @@ int32_t nn_profile_mask(nn_ctx* ctx, uint32_t mask) {
 int32_t nn_kernel_times(nn_ctx* ctx, double* ms, int64_t* launches, int32_t n) {
   if (!ctx) return NN_E_ARG;
   return copy_times(ctx->batch->timer, ms, launches, n);
+}
+
+int32_t nn_kernel_launches(nn_ctx* ctx, int64_t* seen, int32_t n) {
+  if (!ctx) return NN_E_ARG;
+  return copy_seen(ctx->batch->timer, seen, n);
 }
 
 int32_t nn_set_block(nn_ctx* ctx, int32_t block) {
@@ -294,7 +306,24 @@ struct nn_pool {
   int kblock[nn::K_COUNT] = {};
   int kdepth[nn::K_COUNT] = {};
   std::vector<int> member_slot;  // batch index -> slot
+  // device memory no longer used by the batch, freed by the next admission
+  // (hipFree waits for the device, so not on the thread that ticks)
+  std::vector<std::unique_ptr<Batch>> graveyard;
+  std::vector<void*> dead_arrays;
 };
+
+// frees what the ticks left behind; called outside the pool lock
+static void pool_free_dead(nn_pool* p) {
+  std::vector<std::unique_ptr<Batch>> dead;
+  std::vector<void*> arrays;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    dead.swap(p->graveyard);
+    arrays.swap(p->dead_arrays);
+  }
+  dead.clear();
+  for (void* a : arrays) (void)hipFree(a);
+}
 
 extern "C" {
 
@@ -311,6 +340,7 @@ int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) {
   }
   p->budget = budget_bytes;
   p->main.reset(new Batch(p->stream));
+  p->main->defer_free_ = true;
   p->slots.resize(POOL_CAPACITY);
   *out = p;
   return NN_OK;
@@ -318,6 +348,8 @@ int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) {
 
 int32_t nn_pool_destroy(nn_pool* p) {
   if (!p) return NN_E_ARG;
+  if (p->main) p->main->take_deferred(p->dead_arrays);
+  pool_free_dead(p);
   p->main.reset();
   for (auto& s : p->slots) {
     s.staging.reset();
@@ -343,6 +375,7 @@ int64_t nn_pool_footprint(int64_t n, int64_t n_elastic, const nn_params* params)
 int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const float* v, const int32_t* kind,
                       int64_t n, int32_t* slot) {
   if (!p || !params || !x || !kind || n <= 0 || !slot) return NN_E_ARG;
+  pool_free_dead(p);
   auto sc = std::make_shared<SceneHost>();
   sc->params = *params;
   int rc = count_kinds(*sc, kind, n);
@@ -418,40 +451,57 @@ int32_t nn_pool_retire(nn_pool* p, int32_t slot) {
   return NN_OK;
 }
 
-// membership changes are applied at tick boundaries: retired scenes leave,
-// staged scenes join, the batch is re-laid out with state copied over
+// Membership changes are applied at tick boundaries: retired scenes leave,
+// staged scenes join.  Normally nothing moves: the scenes kept stay where they
+// are (a retired scene leaves an unused range) and new scenes are appended
+// after the last used range, so a change costs the new scenes' state copies
+// and the tables.  Only when the room runs out is the batch laid out afresh,
+// packed, with slack for later appends.
 static int pool_apply_membership(nn_pool* p) {
   std::vector<std::shared_ptr<SceneHost>> members;
   std::vector<int> mslot;
   std::vector<const Batch*> from{p->main.get()};
   bool changed = false;
-  for (int k = 0; k < POOL_CAPACITY; k++) {
-    PoolSlot& s = p->slots[k];
-    if (!s.used) continue;
-    if (s.retired) {
+  for (size_t b = 0; b < p->main->scenes.size(); b++) {  // present members, in batch order
+    const int k = p->member_slot[b];
+    if (p->slots[k].retired) {
       changed = true;
       continue;
     }
-    if (!s.scene) continue;  // admission still running
-    if (s.staging) {
-      changed = true;
-      from.push_back(s.staging.get());
-    }
+    members.push_back(p->slots[k].scene);
+    mslot.push_back(k);
+  }
+  for (int k = 0; k < POOL_CAPACITY; k++) {  // staged scenes join at the end
+    PoolSlot& s = p->slots[k];
+    if (!s.used || s.retired || !s.scene || !s.staging) continue;
+    changed = true;
+    from.push_back(s.staging.get());
     members.push_back(s.scene);
     mslot.push_back(k);
   }
+  for (int k = 0; k < POOL_CAPACITY; k++)  // retired before joining
+    if (p->slots[k].used && p->slots[k].retired && p->slots[k].staging) changed = true;
   if (!changed) return NN_OK;
   p->main->block = p->block;
   std::copy(p->kblock, p->kblock + nn::K_COUNT, p->main->kblock);
   std::copy(p->kdepth, p->kdepth + nn::K_COUNT, p->main->kdepth);
-  p->main->scenes = members;
-  int rc = p->main->layout(from);
+  int rc = p->main->append_in_place(members, from);
+  if (rc == Batch::LAYOUT_NEEDED) {
+    long long live = 0;
+    for (int k : mslot) live += p->slots[k].bytes;
+    // room for appends, within the budget
+    const double slack = std::min(1.5, std::max(1.0, 0.9 * (double)p->budget / (double)std::max(live, 1LL)));
+    std::vector<std::shared_ptr<SceneHost>> before = p->main->scenes;
+    p->main->scenes = members;
+    rc = p->main->layout(from, slack, POOL_CAPACITY);
+    if (rc < 0) p->main->scenes = before;  // layout() kept the old arrays
+  }
   if (rc < 0) return rc;
   for (int k = 0; k < POOL_CAPACITY; k++) {
     PoolSlot& s = p->slots[k];
     if (!s.used) continue;
     if (s.retired) {
-      s.staging.reset();
+      if (s.staging) p->graveyard.push_back(std::move(s.staging));
       s.scene.reset();
       s.used = false;
       s.retired = false;
@@ -459,8 +509,9 @@ static int pool_apply_membership(nn_pool* p) {
       s.bytes = 0;
       continue;
     }
-    if (s.staging && s.scene) s.staging.reset();
+    if (s.staging && s.scene) p->graveyard.push_back(std::move(s.staging));
   }
+  p->main->take_deferred(p->dead_arrays);
   p->member_slot = mslot;
   return NN_OK;
 }
@@ -527,13 +578,14 @@ int32_t nn_pool_info(nn_pool* p, int32_t* resident, int32_t* pending, int64_t* b
   return NN_OK;
 }
 
-int32_t nn_pool_profile(nn_pool* p, int32_t enable) { return nn_pool_profile_mask(p, enable ? 0xFFFFFFFFu : 0u); }
+int32_t nn_pool_profile(nn_pool* p, int32_t enable) { return nn_pool_profile_mask(p, enable ? 0xFFFFFFFFu : 0u, 1); }
 
-int32_t nn_pool_profile_mask(nn_pool* p, uint32_t mask) {
-  if (!p) return NN_E_ARG;
+int32_t nn_pool_profile_mask(nn_pool* p, uint32_t mask, int32_t stride) {
+  if (!p || stride < 1) return NN_E_ARG;
   std::lock_guard<std::mutex> lk(p->mu);
   p->main->timer.on = mask != 0;
   p->main->timer.mask = mask;
+  p->main->timer.stride = stride;
   p->main->timer.reset();
   return NN_OK;
 }
@@ -542,6 +594,12 @@ int32_t nn_pool_kernel_times(nn_pool* p, double* ms, int64_t* launches, int32_t 
   if (!p) return NN_E_ARG;
   std::lock_guard<std::mutex> lk(p->mu);
   return copy_times(p->main->timer, ms, launches, n);
+}
+
+int32_t nn_pool_kernel_launches(nn_pool* p, int64_t* seen, int32_t n) {
+  if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return copy_seen(p->main->timer, seen, n);
 }
 
 int32_t nn_pool_set_block(nn_pool* p, int32_t block) {

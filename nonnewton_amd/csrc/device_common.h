@@ -19,7 +19,7 @@ __device__ __forceinline__ bool slot_scene(const Dev& d, int& i, int& s) {
   int w = uniform(i >> 6);
   if (w >= d.W) return false;
   s = uniform(d.wave_scene[w]);
-  return true;
+  return s >= 0;
 }
 
 __device__ __forceinline__ int kind_of(int info) { return info >> 28; }
@@ -163,7 +163,7 @@ __device__ __forceinline__ bool quad_slot_scene(const Dev& d, int& i, int& q, in
   const int w = hw >> 2;
   if (w >= d.W) return false;
   s = uniform(d.wave_scene[w]);
-  return true;
+  return s >= 0;
 }
 
 // (S0 + S1) + (S2 + S3) over the four lanes of a particle; all four lanes end

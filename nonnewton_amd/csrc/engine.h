@@ -82,11 +82,16 @@ struct SceneState {
 
 // Device pointers of a batch (passed by value to every kernel).
 struct Dev {
-  int P;        // particle slots (multiple of 64)
-  int W;        // waves = P / 64
+  int P;        // particle slots allocated (multiple of 64; the stride of per-pair arrays)
+  int W;        // waves allocated = P / 64
+  int Pn;       // slots in use: sweeps are launched over [0, Pn)
   int C;        // total cells
-  int NS;       // stress-set slots (each scene's range padded to 64)
-  int SW;       // stress-set waves = NS / 64
+  int NS;       // stress-set slots allocated (each scene's range padded to 64; the rest-list stride)
+  int SW;       // stress-set waves allocated = NS / 64
+  int NSn;      // stress-set slots in use
+  // A wave (or stress-set wave) that belongs to no scene -- the range of a
+  // retired scene, or slack past the last one -- has scene index -1 and is
+  // skipped by every sweep.
   int nscenes;
   const SceneDev* scenes;
   SceneState* state;

@@ -566,7 +566,7 @@ __global__ void k_advect(Dev d) {
   d.vel[i] = v;
 }
 
-static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.P + block - 1) / block); }
+static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.Pn + block - 1) / block); }
 
 void launch_step_begin(const Dev& d, hipStream_t st) {
   hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, d);

@@ -36,6 +36,7 @@ __device__ __forceinline__ bool member(const Dev& d, int& q, int& s) {
   const int w = uniform(q >> 6);
   if (w >= d.SW) return false;
   s = uniform(d.s_wave_scene[w]);
+  if (s < 0) return false;
   return d.s_pid[q] >= 0;
 }
 
@@ -175,15 +176,15 @@ __global__ void k_elastic_force(Dev d) {
 }
 
 void launch_elastic_init(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_elastic_init, dim3((d.P + block - 1) / block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_elastic_init, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
 }
 void launch_elastic_rot_stress(const Dev& d, int block, hipStream_t st) {
-  if (d.NS <= 0) return;
-  hipLaunchKernelGGL(k_elastic_rot_stress, dim3((d.NS + block - 1) / block), dim3(block), 0, st, d);
+  if (d.NSn <= 0) return;
+  hipLaunchKernelGGL(k_elastic_rot_stress, dim3((d.NSn + block - 1) / block), dim3(block), 0, st, d);
 }
 void launch_elastic_force(const Dev& d, int block, hipStream_t st) {
-  if (d.NS <= 0) return;
-  hipLaunchKernelGGL(k_elastic_force, dim3((d.NS + block - 1) / block), dim3(block), 0, st, d);
+  if (d.NSn <= 0) return;
+  hipLaunchKernelGGL(k_elastic_force, dim3((d.NSn + block - 1) / block), dim3(block), 0, st, d);
 }
 
 }  // namespace nn

@@ -293,19 +293,19 @@ __global__ void k_bvol(Dev d, int dynamic) {
 }
 
 void launch_cell_count(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_count, dim3((d.P + block - 1) / block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_cell_count, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
 }
 void launch_bucket(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket, dim3((d.P + block - 1) / block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_bucket, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
 }
 void launch_place(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_place, dim3((d.P + block - 1) / block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_place, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
 }
 void launch_build_lists(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_lists, dim3((d.P + block - 1) / block), dim3(block), 0, st, d);
+  hipLaunchKernelGGL(k_build_lists, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
 }
 void launch_bvol(const Dev& d, int block, int dynamic, hipStream_t st) {
-  hipLaunchKernelGGL(k_bvol, dim3((d.P + block - 1) / block), dim3(block), 0, st, d, dynamic);
+  hipLaunchKernelGGL(k_bvol, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d, dynamic);
 }
 
 }  // namespace nn

@@ -337,13 +337,13 @@ __global__ void k_cg_update(Dev d) {
   wave_partials<2>(d, (int)(i >> 6), v);
 }
 
-static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.P + block - 1) / block); }
+static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.Pn + block - 1) / block); }
 // four-lane sweeps: a workgroup covers the particles of a one-lane workgroup
 // (fewer workgroups to dispatch when most scenes have left the solve)
 static inline int quad_block(int block) { return block * 4 <= 1024 ? block * 4 : 1024; }
 static inline dim3 grid4_of(const Dev& d, int block) {
   const int qb = quad_block(block);
-  return dim3((4 * d.P + qb - 1) / qb);
+  return dim3((4 * d.Pn + qb - 1) / qb);
 }
 
 void launch_strain(const Dev& d, int block, int depth, hipStream_t st) {

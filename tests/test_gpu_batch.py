@@ -46,6 +46,31 @@ def test_pool_equals_alone_and_churn():
     pool.close()
 
 
+def test_pool_holes_appends_and_compaction():
+    """Membership changes without moving scenes: a retired scene leaves a hole,
+    a new one is appended after the last used range; a scene larger than the
+    remaining room forces a packed layout.  Every scene stays equal to itself
+    run alone."""
+    small = scenes.mixed_batch(count=4, lo=2000, hi=3000)
+    big = scenes.mixed_scene(np.random.Generator(np.random.PCG64(5)), 30000, 2, 5.0, 1.0, True, "big")
+    pool = L.Pool(16 << 30)
+    s0, s1, s2 = (pool.admit(sc.params, sc.x, sc.kind, sc.v) for sc in small[:3])
+    pool.tick()                      # packed layout of 3 scenes, with room to spare
+    pool.retire(s1)
+    s3 = pool.admit(small[3].params, small[3].x, small[3].kind, small[3].v)
+    pool.tick()                      # hole where s1 was, small[3] appended
+    sb = pool.admit(big.params, big.x, big.kind, big.v)
+    pool.tick()                      # does not fit the room: packed layout again
+    pool.tick()
+    assert pool.info()[:2] == (4, 0)
+    for sc, slot, steps in ((small[0], s0, 4), (small[2], s2, 4), (small[3], s3, 3), (big, sb, 2)):
+        ref, st = alone(sc, steps)
+        for f, a in ref.items():
+            assert np.array_equal(pool.read(slot, f), a), (sc.name, f)
+        assert pool.steps(slot) == steps
+    pool.close()
+
+
 def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
     """Every kernel at 64- and 1024-thread workgroups with the largest and the
     smallest gather depths, and with autotuned shapes: the same bits as the
@@ -109,3 +134,24 @@ def test_pool_budget():
     fp = big.footprint(sc.params, sc.n)
     assert 0 < fp < (1 << 32)
     assert big.admit(sc.params, sc.x, sc.kind) is not None
+
+
+def test_sampled_kernel_timing():
+    """Events around every stride-th launch of the chosen kernel only: the timed
+    count is ceil(launches / stride), every launch is counted, no other kernel
+    is timed."""
+    batch = scenes.mixed_batch(count=2, lo=2000, hi=3000)
+    pool = L.Pool(4 << 30)
+    for s in batch:
+        pool.admit(s.params, s.x, s.kind, s.v)
+    pool.tick()
+    pool.profile(True, kernels=["cg_matvec"], stride=3)
+    for _ in range(3):
+        pool.tick()
+    times = pool.kernel_times()
+    seen = pool.kernel_launches()
+    ms, timed = times["cg_matvec"]
+    assert seen["cg_matvec"] >= 6 and timed == (seen["cg_matvec"] + 2) // 3 and ms > 0
+    assert all(n == 0 for k, (_, n) in times.items() if k != "cg_matvec")
+    pool.profile(False)
+    pool.close()

@@ -180,6 +180,7 @@ Batch::Batch(hipStream_t st) : stream(st) {}
 
 Batch::~Batch() {
   release();
+  if (dev_slots_) (void)hipFree(dev_slots_);
   if (h_state_) (void)hipHostFree(h_state_);
   if (h_active_) (void)hipHostFree(h_active_);
 }
@@ -205,6 +206,24 @@ void Batch::release() {
   for (auto& a : allocs_) (void)hipFree(a.p);
   allocs_.clear();
   bytes_ = 0;
+}
+
+// The device copy of the current Dev.  The batch cycles through a few Dev
+// values (the sort and the non-pressure sweep swap buffer pairs), so copies
+// are cached in DEV_SLOTS device slots and uploaded only when a value is new.
+// A slot is reused round-robin, after a stream synchronization once per lap,
+// so an upload never overwrites a slot a queued launch may still read.
+DevRef Batch::dref() {
+  for (int k = 0; k < DEV_SLOTS; k++)
+    if (slot_used_[k] && std::memcmp(&slot_host_[k], &d, sizeof(Dev)) == 0) return DevRef{d, dev_slots_ + k};
+  const int k = next_slot_;
+  next_slot_ = (next_slot_ + 1) % DEV_SLOTS;
+  if (next_slot_ == 0) (void)hipStreamSynchronize(stream);
+  std::memcpy(&slot_host_[k], &d, sizeof(Dev));
+  slot_used_[k] = true;
+  hipError_t e = hipMemcpyAsync(dev_slots_ + k, &slot_host_[k], sizeof(Dev), hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) set_error(std::string("Dev upload: ") + hipGetErrorString(e));
+  return DevRef{d, dev_slots_ + k};
 }
 
 void Batch::swap_sort_buffers() {
@@ -416,6 +435,7 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   C_cap_ = Cc;
   scene_cap_ = Sc;
   scan_blocks_ = nscan;
+  if (!dev_slots_) HIPCHK(hipMalloc((void**)&dev_slots_, sizeof(Dev) * DEV_SLOTS));
   if (!h_state_) {
     HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * 4096, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
@@ -623,18 +643,18 @@ void KernelTimer::reset() {
 
 int Batch::search_impl() {
   launch_fill_i32(d.cell_count, (long)C_total_ + 1, 0, stream);
-  TIMED(K_CELL_COUNT, launch_cell_count(d, blk(K_CELL_COUNT), stream));
-  TIMED(K_SCAN, launch_scan(d, C_total_ + 1, stream));
-  TIMED(K_BUCKET, launch_bucket(d, blk(K_BUCKET), stream));
-  TIMED(K_PLACE, launch_place(d, blk(K_PLACE), stream));
+  TIMED(K_CELL_COUNT, launch_cell_count(dref(), blk(K_CELL_COUNT), stream));
+  TIMED(K_SCAN, launch_scan(dref(), C_total_ + 1, stream));
+  TIMED(K_BUCKET, launch_bucket(dref(), blk(K_BUCKET), stream));
+  TIMED(K_PLACE, launch_place(dref(), blk(K_PLACE), stream));
   swap_sort_buffers();
-  TIMED(K_BUILD_LISTS, launch_build_lists(d, blk(K_BUILD_LISTS), stream));
+  TIMED(K_BUILD_LISTS, launch_build_lists(dref(), blk(K_BUILD_LISTS), stream));
   return NN_OK;
 }
 
 int Batch::search() {
   if (scenes.empty()) return NN_OK;
-  launch_step_begin(d, stream);
+  launch_step_begin(dref(), stream);
   CHK(search_impl());
   HIPCHK(hipGetLastError());
   CHK(gather_state());
@@ -733,9 +753,9 @@ int Batch::build_stress_set(const std::vector<int>& which) {
 
 int Batch::init_scenes(const std::vector<int>& which) {
   if (scenes.empty()) return NN_OK;
-  launch_step_begin(d, stream);
+  launch_step_begin(dref(), stream);
   CHK(search_impl());
-  launch_elastic_init(d, block, stream);
+  launch_elastic_init(dref(), block, stream);
   HIPCHK(hipGetLastError());
   CHK(gather_state());
   for (size_t k = 0; k < scenes.size(); k++) {
@@ -748,8 +768,8 @@ int Batch::init_scenes(const std::vector<int>& which) {
   // the stress-set relayout does not carry neighbor lists (they hold absolute
   // slots); search again on the unchanged positions before the volume sweeps
   CHK(search_impl());
-  launch_bvol(d, block, 0, stream);
-  launch_bvol(d, block, 1, stream);
+  launch_bvol(dref(), block, 0, stream);
+  launch_bvol(dref(), block, 1, stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
   for (int k : which) scenes[k]->initialized = true;
@@ -767,17 +787,17 @@ int Batch::solver_loop(int which) {
   for (;;) {
     for (int c = 0; c < chunk; c++) {
       if (which == 0) {
-        TIMED(K_DIV_UPDATE, launch_div_update(d, blk(K_DIV_UPDATE), dep(K_DIV_UPDATE), stream));
-        TIMED(K_DIV_SOURCE, launch_div_source(d, blk(K_DIV_SOURCE), dep(K_DIV_SOURCE), stream));
-        launch_scene_finish(d, 0, stream);
+        TIMED(K_DIV_UPDATE, launch_div_update(dref(), blk(K_DIV_UPDATE), dep(K_DIV_UPDATE), stream));
+        TIMED(K_DIV_SOURCE, launch_div_source(dref(), blk(K_DIV_SOURCE), dep(K_DIV_SOURCE), stream));
+        launch_scene_finish(dref(), 0, stream);
       } else if (which == 1) {
-        TIMED(K_PRESS_UPDATE, launch_press_update(d, blk(K_PRESS_UPDATE), dep(K_PRESS_UPDATE), stream));
-        TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 1, stream));
-        launch_scene_finish(d, 1, stream);
+        TIMED(K_PRESS_UPDATE, launch_press_update(dref(), blk(K_PRESS_UPDATE), dep(K_PRESS_UPDATE), stream));
+        TIMED(K_PRESS_SOURCE, launch_press_source(dref(), blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 1, stream));
+        launch_scene_finish(dref(), 1, stream);
       } else {
-        TIMED(K_CG_UPDATE, launch_cg_update(d, blk(K_CG_UPDATE), stream));
-        TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
-        launch_scene_finish(d, 3, stream);
+        TIMED(K_CG_UPDATE, launch_cg_update(dref(), blk(K_CG_UPDATE), stream));
+        TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
+        launch_scene_finish(dref(), 3, stream);
       }
     }
     done += chunk;
@@ -797,37 +817,37 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
       set_error("step before init");
       return NN_E_STATE;
     }
-  launch_step_begin(d, stream);
+  launch_step_begin(dref(), stream);
   CHK(search_impl());                                                  // :917
-  if (any_elastic_) TIMED(K_BVOL, launch_bvol(d, blk(K_BVOL), 1, stream));  // :918
-  TIMED(K_CLEAR_ACC, launch_clear_acc(d, blk(K_CLEAR_ACC), stream));             // :920
+  if (any_elastic_) TIMED(K_BVOL, launch_bvol(dref(), blk(K_BVOL), 1, stream));  // :918
+  TIMED(K_CLEAR_ACC, launch_clear_acc(dref(), blk(K_CLEAR_ACC), stream));             // :920
   if (any_fluid_) {  // DFSPHSolver.substep, :1546-1555
-    TIMED(K_DENSITY_ALPHA, launch_density_alpha(d, blk(K_DENSITY_ALPHA), dep(K_DENSITY_ALPHA), stream));
+    TIMED(K_DENSITY_ALPHA, launch_density_alpha(dref(), blk(K_DENSITY_ALPHA), dep(K_DENSITY_ALPHA), stream));
     if (any_div_) {
-      launch_solver_begin(d, 0, stream);
+      launch_solver_begin(dref(), 0, stream);
       CHK(solver_loop(0));
     }
-    if (any_strain_) TIMED(K_STRAIN, launch_strain(d, blk(K_STRAIN), dep(K_STRAIN), stream));
+    if (any_strain_) TIMED(K_STRAIN, launch_strain(dref(), blk(K_STRAIN), dep(K_STRAIN), stream));
     if (any_weiler_) {
-      launch_solver_begin(d, 2, stream);
-      TIMED(K_CG_INIT, launch_cg_init(d, blk(K_CG_INIT), stream));
-      if (any_warm_cg_) launch_scene_finish(d, 4, stream);
-      TIMED(K_CG_GUESS, launch_cg_guess(d, blk(K_CG_GUESS), stream));
-      TIMED(K_CG_MATVEC, launch_cg_matvec(d, blk(K_CG_MATVEC), dep(K_CG_MATVEC), 1, stream));
-      launch_scene_finish(d, 2, stream);
+      launch_solver_begin(dref(), 2, stream);
+      TIMED(K_CG_INIT, launch_cg_init(dref(), blk(K_CG_INIT), stream));
+      if (any_warm_cg_) launch_scene_finish(dref(), 4, stream);
+      TIMED(K_CG_GUESS, launch_cg_guess(dref(), blk(K_CG_GUESS), stream));
+      TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 1, stream));
+      launch_scene_finish(dref(), 2, stream);
       CHK(solver_loop(2));
     }
-    TIMED(K_NONPRESSURE, launch_nonpressure(d, blk(K_NONPRESSURE), dep(K_NONPRESSURE), stream));
+    TIMED(K_NONPRESSURE, launch_nonpressure(dref(), blk(K_NONPRESSURE), dep(K_NONPRESSURE), stream));
     std::swap(d.vel, d.vel_alt);
-    launch_solver_begin(d, 1, stream);
-    TIMED(K_PRESS_SOURCE, launch_press_source(d, blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 0, stream));
+    launch_solver_begin(dref(), 1, stream);
+    TIMED(K_PRESS_SOURCE, launch_press_source(dref(), blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 0, stream));
     CHK(solver_loop(1));
   }
   if (any_elastic_) {  // Elasticity.substep, :1241-1244
-    TIMED(K_ELASTIC_ROT_STRESS, launch_elastic_rot_stress(d, blk(K_ELASTIC_ROT_STRESS), stream));
-    TIMED(K_ELASTIC_FORCE, launch_elastic_force(d, blk(K_ELASTIC_FORCE), stream));
+    TIMED(K_ELASTIC_ROT_STRESS, launch_elastic_rot_stress(dref(), blk(K_ELASTIC_ROT_STRESS), stream));
+    TIMED(K_ELASTIC_FORCE, launch_elastic_force(dref(), blk(K_ELASTIC_FORCE), stream));
   }
-  TIMED(K_ADVECT, launch_advect(d, blk(K_ADVECT), stream));  // :933, :936-937
+  TIMED(K_ADVECT, launch_advect(dref(), blk(K_ADVECT), stream));  // :933, :936-937
   HIPCHK(hipGetLastError());
   CHK(gather_state());
   CHK(timer.collect());

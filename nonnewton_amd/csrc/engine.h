@@ -144,34 +144,44 @@ struct Dev {
 // ---------------------------------------------------------------------------
 // host-side launchers (kernels_*.hip)
 // ---------------------------------------------------------------------------
+// What a launcher needs of a batch: the host copy of its Dev (grid sizes) and
+// the address of the same Dev resident in device memory, which is the kernels'
+// only argument besides a mode flag.  A by-value Dev (about 600 bytes of
+// kernel arguments) roughly doubles the time a dependent launch takes to
+// start on this device; a pointer is 8 bytes.
+struct DevRef {
+  const Dev& h;
+  const Dev* g;
+};
+
 void launch_fill_i32(int* p, long n, int v, hipStream_t st);
 void launch_fill_f4(float4* p, long n, float4 v, hipStream_t st);
-void launch_scan(const Dev& d, int n, hipStream_t st);  // cell_start = exclusive scan(cell_count)
+void launch_scan(const DevRef& d, int n, hipStream_t st);  // cell_start = exclusive scan(cell_count)
 
-void launch_cell_count(const Dev& d, int block, hipStream_t st);
-void launch_bucket(const Dev& d, int block, hipStream_t st);
-void launch_place(const Dev& d, int block, hipStream_t st);
-void launch_build_lists(const Dev& d, int block, hipStream_t st);
-void launch_bvol(const Dev& d, int block, int dynamic, hipStream_t st);
+void launch_cell_count(const DevRef& d, int block, hipStream_t st);
+void launch_bucket(const DevRef& d, int block, hipStream_t st);
+void launch_place(const DevRef& d, int block, hipStream_t st);
+void launch_build_lists(const DevRef& d, int block, hipStream_t st);
+void launch_bvol(const DevRef& d, int block, int dynamic, hipStream_t st);
 
-void launch_density_alpha(const Dev& d, int block, int depth, hipStream_t st);
-void launch_div_update(const Dev& d, int block, int depth, hipStream_t st);
-void launch_div_source(const Dev& d, int block, int depth, hipStream_t st);
-void launch_solver_begin(const Dev& d, int which, hipStream_t st);
-void launch_strain(const Dev& d, int block, int depth, hipStream_t st);
-void launch_cg_init(const Dev& d, int block, hipStream_t st);
-void launch_cg_guess(const Dev& d, int block, hipStream_t st);
-void launch_cg_matvec(const Dev& d, int block, int depth, int start, hipStream_t st);  // depth 1, 2, 4, 8
-void launch_cg_update(const Dev& d, int block, hipStream_t st);
-void launch_nonpressure(const Dev& d, int block, int depth, hipStream_t st);
-void launch_press_source(const Dev& d, int block, int depth, int reduce, hipStream_t st);
-void launch_press_update(const Dev& d, int block, int depth, hipStream_t st);
-void launch_clear_acc(const Dev& d, int block, hipStream_t st);
-void launch_elastic_init(const Dev& d, int block, hipStream_t st);
-void launch_elastic_rot_stress(const Dev& d, int block, hipStream_t st);
-void launch_elastic_force(const Dev& d, int block, hipStream_t st);
-void launch_advect(const Dev& d, int block, hipStream_t st);
-void launch_step_begin(const Dev& d, hipStream_t st);
-void launch_scene_finish(const Dev& d, int which, hipStream_t st);  // 0 div, 1 press, 2 cg start, 3 cg iteration, 4 cg guess scale
+void launch_density_alpha(const DevRef& d, int block, int depth, hipStream_t st);
+void launch_div_update(const DevRef& d, int block, int depth, hipStream_t st);
+void launch_div_source(const DevRef& d, int block, int depth, hipStream_t st);
+void launch_solver_begin(const DevRef& d, int which, hipStream_t st);
+void launch_strain(const DevRef& d, int block, int depth, hipStream_t st);
+void launch_cg_init(const DevRef& d, int block, hipStream_t st);
+void launch_cg_guess(const DevRef& d, int block, hipStream_t st);
+void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStream_t st);  // depth 1, 2, 4, 8
+void launch_cg_update(const DevRef& d, int block, hipStream_t st);
+void launch_nonpressure(const DevRef& d, int block, int depth, hipStream_t st);
+void launch_press_source(const DevRef& d, int block, int depth, int reduce, hipStream_t st);
+void launch_press_update(const DevRef& d, int block, int depth, hipStream_t st);
+void launch_clear_acc(const DevRef& d, int block, hipStream_t st);
+void launch_elastic_init(const DevRef& d, int block, hipStream_t st);
+void launch_elastic_rot_stress(const DevRef& d, int block, hipStream_t st);
+void launch_elastic_force(const DevRef& d, int block, hipStream_t st);
+void launch_advect(const DevRef& d, int block, hipStream_t st);
+void launch_step_begin(const DevRef& d, hipStream_t st);
+void launch_scene_finish(const DevRef& d, int which, hipStream_t st);  // 0 div, 1 press, 2 cg start, 3 cg iteration, 4 cg guess scale
 
 }  // namespace nn

@@ -18,7 +18,8 @@ __device__ __forceinline__ float alpha_press(const SceneDev& sc, float a) {
   return (sc.flags & SF_DIVERGENCE) ? ((a * sc.inv_dt) * sc.dt) * sc.inv_dt2 : a * sc.inv_dt2;
 }
 
-__global__ void k_step_begin(Dev d) {
+__global__ void k_step_begin(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   for (int s = threadIdx.x; s < d.nscenes; s += blockDim.x) {
     d.state[s].status = 0;
     d.state[s].max_rho_bits = 0u;
@@ -29,7 +30,8 @@ __global__ void k_step_begin(Dev d) {
 }
 
 // clear_accelerations_kernel, :945-951
-__global__ void k_clear_acc(Dev d) {
+__global__ void k_clear_acc(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -42,7 +44,8 @@ __global__ void k_clear_acc(Dev d) {
 // compute_density_change of the divergence solve (:1610-1645), fused: all three
 // read the same neighbors and none reads what another writes.
 template <int B>
-__global__ void k_density_alpha(Dev d) {
+__global__ void k_density_alpha(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -91,7 +94,8 @@ __global__ void k_density_alpha(Dev d) {
 }
 
 // which: 0 divergence, 1 pressure, 2 cg (cg_init activates scenes)
-__global__ void k_solver_begin(Dev d, int which) {
+__global__ void k_solver_begin(const Dev* __restrict__ dp, int which) {
+  const Dev& d = *dp;
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -127,7 +131,8 @@ __global__ void k_solver_begin(Dev d, int which) {
 // alpha, writes v).  Reactions on dynamic solids are not formed: the second
 // clear_accelerations (:1552) erases them.
 template <int B>
-__global__ void k_div_update(Dev d) {
+__global__ void k_div_update(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -191,7 +196,8 @@ __device__ __forceinline__ float velocity_divergence(const Dev& d, const SceneDe
 // compute_density_change (:1610-1645) + compute_density_error(0) (:1669-1675)
 // and the loop test of divergence_solve (:1695-1703)
 template <int B>
-__global__ void k_div_source(Dev d) {
+__global__ void k_div_source(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -215,7 +221,8 @@ __global__ void k_div_source(Dev d) {
 // clear_accelerations (:1552) is folded in.  v is double-buffered: the sweep
 // reads vel and writes vel_alt.
 template <int B>
-__global__ void k_nonpressure(Dev d) {
+__global__ void k_nonpressure(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -333,7 +340,8 @@ __global__ void k_nonpressure(Dev d) {
 // compute_density_adv (:1647-1667) + compute_density_error(rho0) and the loop
 // test of pressure_solve (:1778-1784)
 template <int B>
-__global__ void k_press_source(Dev d, int reduce) {
+__global__ void k_press_source(const Dev* __restrict__ dp, int reduce) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -361,7 +369,8 @@ __global__ void k_press_source(Dev d, int reduce) {
 // combined as (W0 + W1) + (W2 + W3): a fixed order, the same bits every run.
 constexpr int FINISH_THREADS = 256;
 
-__global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int which) {
+__global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __restrict__ dp, int which) {
+  const Dev& d = *dp;
   __shared__ double red[4][FINISH_THREADS / 64];
   const int s = blockIdx.x;
   const int t = threadIdx.x;
@@ -460,7 +469,8 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int whic
 // (nothing in the sweep reads v); dynamic-solid lanes gather the reaction
 // -vel_change/dt of every fluid neighbor (:1835-1841) into their acceleration.
 template <int B>
-__global__ void k_press_update(Dev d) {
+__global__ void k_press_update(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -523,7 +533,8 @@ __global__ void k_press_update(Dev d) {
 
 // advect (:1086-1093) then enforce_boundary_3D for FLUID and SOLID (:1044-1080,
 // :936-937), fused: each dynamic particle is touched once.
-__global__ void k_advect(Dev d) {
+__global__ void k_advect(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -566,40 +577,40 @@ __global__ void k_advect(Dev d) {
   d.vel[i] = v;
 }
 
-static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.Pn + block - 1) / block); }
+static inline dim3 grid_of(const DevRef& d, int block) { return dim3((d.h.Pn + block - 1) / block); }
 
-void launch_step_begin(const Dev& d, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, d);
+void launch_step_begin(const DevRef& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, d.g);
 }
-void launch_clear_acc(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d);
+void launch_clear_acc(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d.g);
 }
-void launch_density_alpha(const Dev& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_density_alpha, depth, grid_of(d, block), dim3(block), st, d);
+void launch_density_alpha(const DevRef& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_density_alpha, depth, grid_of(d, block), dim3(block), st, d.g);
 }
-void launch_solver_begin(const Dev& d, int which, hipStream_t st) {
-  hipLaunchKernelGGL(k_solver_begin, dim3(1), dim3(256), 0, st, d, which);
+void launch_solver_begin(const DevRef& d, int which, hipStream_t st) {
+  hipLaunchKernelGGL(k_solver_begin, dim3(1), dim3(256), 0, st, d.g, which);
 }
-void launch_div_update(const Dev& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_div_update, depth, grid_of(d, block), dim3(block), st, d);
+void launch_div_update(const DevRef& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_div_update, depth, grid_of(d, block), dim3(block), st, d.g);
 }
-void launch_div_source(const Dev& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_div_source, depth, grid_of(d, block), dim3(block), st, d);
+void launch_div_source(const DevRef& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_div_source, depth, grid_of(d, block), dim3(block), st, d.g);
 }
-void launch_nonpressure(const Dev& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_nonpressure, depth, grid_of(d, block), dim3(block), st, d);
+void launch_nonpressure(const DevRef& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_nonpressure, depth, grid_of(d, block), dim3(block), st, d.g);
 }
-void launch_press_source(const Dev& d, int block, int depth, int reduce, hipStream_t st) {
-  DEPTH_LAUNCH(k_press_source, depth, grid_of(d, block), dim3(block), st, d, reduce);
+void launch_press_source(const DevRef& d, int block, int depth, int reduce, hipStream_t st) {
+  DEPTH_LAUNCH(k_press_source, depth, grid_of(d, block), dim3(block), st, d.g, reduce);
 }
-void launch_press_update(const Dev& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_press_update, depth, grid_of(d, block), dim3(block), st, d);
+void launch_press_update(const DevRef& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_press_update, depth, grid_of(d, block), dim3(block), st, d.g);
 }
-void launch_advect(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d);
+void launch_advect(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d.g);
 }
-void launch_scene_finish(const Dev& d, int which, hipStream_t st) {
-  if (d.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.nscenes), dim3(FINISH_THREADS), 0, st, d, which);
+void launch_scene_finish(const DevRef& d, int which, hipStream_t st) {
+  if (d.h.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.h.nscenes), dim3(FINISH_THREADS), 0, st, d.g, which);
 }
 
 }  // namespace nn

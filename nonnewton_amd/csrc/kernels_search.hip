@@ -115,11 +115,11 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(const int* in, int n, cons
   }
 }
 
-void launch_scan(const Dev& d, int n, hipStream_t st) {
+void launch_scan(const DevRef& d, int n, hipStream_t st) {
   int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, d.cell_count, n, d.scan_tmp);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, d.scan_tmp, nb);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, st, d.cell_count, n, d.scan_tmp, d.cell_start);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, d.h.cell_count, n, d.h.scan_tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, d.h.scan_tmp, nb);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, st, d.h.cell_count, n, d.h.scan_tmp, d.h.cell_start);
 }
 
 // ---------------------------------------------------------------------------
@@ -140,7 +140,8 @@ __device__ __forceinline__ bool cell_of(const SceneDev& sc, float4 p, int& cx, i
 // the first lane of each run adds the run length with one atomic and the run's
 // lanes take consecutive ranks from it.  (Ranks only need to be unique; the
 // final order inside a cell is fixed by k_place.)
-__global__ void k_cell_count(Dev d) {
+__global__ void k_cell_count(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;  // uniform over the wavefront
   const SceneDev& sc = d.scenes[s];
@@ -174,7 +175,8 @@ __global__ void k_cell_count(Dev d) {
   d.rank[i] = r;
 }
 
-__global__ void k_bucket(Dev d) {
+__global__ void k_bucket(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -182,7 +184,8 @@ __global__ void k_bucket(Dev d) {
   d.bucket[d.cell_start[d.key[i]] + d.rank[i]] = i;
 }
 
-__global__ void k_place(Dev d) {
+__global__ void k_place(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -202,7 +205,8 @@ __global__ void k_place(Dev d) {
 }
 
 // store_neighbors, SPH/main.py:836-847, over the sorted particles
-__global__ void k_build_lists(Dev d) {
+__global__ void k_build_lists(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -265,7 +269,8 @@ __global__ void k_build_lists(Dev d) {
 
 // compute_static/moving_boundary_volume, SPH/main.py:1010-1035:
 // V_b = 3 / (W0 + sum over solid neighbors of W); stored as -V (solid sign).
-__global__ void k_bvol(Dev d, int dynamic) {
+__global__ void k_bvol(const Dev* __restrict__ dp, int dynamic) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -292,20 +297,20 @@ __global__ void k_bvol(Dev d, int dynamic) {
   d.pos[i].w = -(1.0f / delta * 3.0f);
 }
 
-void launch_cell_count(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_count, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
+void launch_cell_count(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_count, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
 }
-void launch_bucket(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
+void launch_bucket(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_bucket, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
 }
-void launch_place(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_place, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
+void launch_place(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_place, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
 }
-void launch_build_lists(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_lists, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d);
+void launch_build_lists(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_lists, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
 }
-void launch_bvol(const Dev& d, int block, int dynamic, hipStream_t st) {
-  hipLaunchKernelGGL(k_bvol, dim3((d.Pn + block - 1) / block), dim3(block), 0, st, d, dynamic);
+void launch_bvol(const DevRef& d, int block, int dynamic, hipStream_t st) {
+  hipLaunchKernelGGL(k_bvol, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g, dynamic);
 }
 
 }  // namespace nn

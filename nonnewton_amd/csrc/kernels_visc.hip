@@ -37,7 +37,8 @@ __device__ __forceinline__ bool weiler_scene(const SceneDev& sc) {
 // NonNewton.calcStrainRate (:461-492, with the accumulation the reference
 // omits) + the viscosity model; mu for the implicit solver.
 template <int B>
-__global__ void k_strain(Dev d) {
+__global__ void k_strain(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -87,7 +88,8 @@ __global__ void k_strain(Dev d) {
 // solve's e = x - v (into ww) and partial sums of (e.r0, e.Ae), from which the
 // scene finish forms theta = (e.r0) / (e.Ae); k_cg_guess then starts CG from
 // x0 = b + theta e, r = r0 - theta A e.
-__global__ void k_cg_init(Dev d) {
+__global__ void k_cg_init(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
@@ -195,7 +197,8 @@ __global__ void k_cg_init(Dev d) {
 
 // x0 = b + theta e, r = r0 - theta A e (x0 = b, r = r0 without SF_WARM_CG),
 // u = M r; partials (b.b, r.r, r.u).
-__global__ void k_cg_guess(Dev d) {
+__global__ void k_cg_guess(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -234,7 +237,8 @@ __global__ void k_cg_guess(Dev d) {
 // them is used (a shorter chain of dependent loads); the terms are added in
 // list order whatever D is.
 template <int D>
-__global__ void k_cg_matvec(Dev d, int start) {
+__global__ void k_cg_matvec(const Dev* __restrict__ dp, int start) {
+  const Dev& d = *dp;
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
@@ -290,7 +294,8 @@ __global__ void k_cg_matvec(Dev d, int start) {
 
 // p = u + beta p, s = w + beta s (p = u, s = w on the first iteration);
 // x += alpha p; r -= alpha s; u = M r; partials (r.r, r.u).
-__global__ void k_cg_update(Dev d) {
+__global__ void k_cg_update(const Dev* __restrict__ dp) {
+  const Dev& d = *dp;
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -337,35 +342,35 @@ __global__ void k_cg_update(Dev d) {
   wave_partials<2>(d, (int)(i >> 6), v);
 }
 
-static inline dim3 grid_of(const Dev& d, int block) { return dim3((d.Pn + block - 1) / block); }
+static inline dim3 grid_of(const DevRef& d, int block) { return dim3((d.h.Pn + block - 1) / block); }
 // four-lane sweeps: a workgroup covers the particles of a one-lane workgroup
 // (fewer workgroups to dispatch when most scenes have left the solve)
 static inline int quad_block(int block) { return block * 4 <= 1024 ? block * 4 : 1024; }
-static inline dim3 grid4_of(const Dev& d, int block) {
+static inline dim3 grid4_of(const DevRef& d, int block) {
   const int qb = quad_block(block);
-  return dim3((4 * d.Pn + qb - 1) / qb);
+  return dim3((4 * d.h.Pn + qb - 1) / qb);
 }
 
-void launch_strain(const Dev& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_strain, depth, grid_of(d, block), dim3(block), st, d);
+void launch_strain(const DevRef& d, int block, int depth, hipStream_t st) {
+  DEPTH_LAUNCH(k_strain, depth, grid_of(d, block), dim3(block), st, d.g);
 }
-void launch_cg_init(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d);
+void launch_cg_init(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d.g);
 }
-void launch_cg_guess(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d);
+void launch_cg_guess(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d.g);
 }
-void launch_cg_matvec(const Dev& d, int block, int depth, int start, hipStream_t st) {
+void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStream_t st) {
   const dim3 g = grid4_of(d, block), b = dim3(quad_block(block));
   switch (depth) {
-    case 1: hipLaunchKernelGGL(k_cg_matvec<1>, g, b, 0, st, d, start); break;
-    case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d, start); break;
-    case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d, start); break;
-    default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d, start); break;
+    case 1: hipLaunchKernelGGL(k_cg_matvec<1>, g, b, 0, st, d.g, start); break;
+    case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d.g, start); break;
+    case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d.g, start); break;
+    default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d.g, start); break;
   }
 }
-void launch_cg_update(const Dev& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d);
+void launch_cg_update(const DevRef& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d.g);
 }
 
 }  // namespace nn

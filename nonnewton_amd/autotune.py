@@ -29,8 +29,9 @@ from typing import Dict, Iterable, Optional
 from . import _lib
 
 CANDIDATES = (64, 128, 256, 512, 1024)
-# kernels with a fixed shape (scan, the scene finish) or launched once at init
-FIXED = {"scan"}
+# kernels with a fixed shape: the scan, and the CG update (scene-aligned
+# 1024-thread workgroups)
+FIXED = {"scan", "cg_update"}
 
 
 def _signature(device: str, scenes: int, particles: int) -> str:

@@ -125,6 +125,7 @@ class Batch {
   int C_total_ = 0;  // cells in use
   int C_cap_ = 0;    // cells allocated
   int scene_cap_ = 0;
+  int n_cg_blocks_ = 0;  // entries of d.cg_blocks
   std::vector<void*> deferred_;
   static constexpr int DEV_SLOTS = 16;
   Dev* dev_slots_ = nullptr;   // device copies of recent Dev values
@@ -143,6 +144,7 @@ class Batch {
   void release();
   int search_impl();
   int solver_loop(int which);
+  int cg_loop();
   int build_stress_set(const std::vector<int>& which);
   int gather_state();
   void swap_sort_buffers();

@@ -270,12 +270,19 @@ void Batch::derive_members() {
 int Batch::upload_tables(const Dev& nd) {
   std::vector<int> hw(std::max(nd.W, 1), -1), hsw(std::max(nd.SW, 1), -1);
   std::vector<SceneDev> hs(scenes.size());
+  std::vector<int2> cgb;
   for (size_t k = 0; k < scenes.size(); k++) {
     SceneHost& s = *scenes[k];
     for (int w = s.dev.wave_begin; w < s.dev.wave_end; w++) hw[w] = (int)k;
     for (int w = s.s_begin / 64; w < (s.s_begin + round_up64(s.ns)) / 64; w++) hsw[w] = (int)k;
     hs[k] = s.dev;
+    const int f = s.dev.flags;
+    if ((f & SF_HAS_FLUID) && (f & SF_VISCOSITY) && (f & SF_WEILER))  // CG update workgroups
+      for (int w = s.dev.wave_begin; w < s.dev.wave_end; w += 16) cgb.push_back(make_int2((int)k, w));
   }
+  n_cg_blocks_ = (int)cgb.size();
+  if (!cgb.empty())
+    HIPCHK(hipMemcpyAsync((void*)nd.cg_blocks, cgb.data(), sizeof(int2) * cgb.size(), hipMemcpyHostToDevice, stream));
   if (!scenes.empty())
     HIPCHK(hipMemcpyAsync((void*)nd.scenes, hs.data(), sizeof(SceneDev) * hs.size(), hipMemcpyHostToDevice, stream));
   if (nd.W) HIPCHK(hipMemcpyAsync((void*)nd.wave_scene, hw.data(), sizeof(int) * nd.W, hipMemcpyHostToDevice, stream));
@@ -398,7 +405,10 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.cell_count, ncell_entries);
   A(nd.cell_start, ncell_entries);
   A(nd.scan_tmp, nscan);
-  A(nd.partials, 16 * (size_t)nd.W);
+  A(nd.partials, 32 * (size_t)nd.W);
+  int2* cgb;
+  A(cgb, (size_t)nd.W / 16 + Sc + 1);
+  nd.cg_blocks = cgb;
   A(nd.n_active, 4);
   A(nd.wx, Pc);
   A(nd.wr, Pc);
@@ -794,10 +804,6 @@ int Batch::solver_loop(int which) {
         TIMED(K_PRESS_UPDATE, launch_press_update(dref(), blk(K_PRESS_UPDATE), dep(K_PRESS_UPDATE), stream));
         TIMED(K_PRESS_SOURCE, launch_press_source(dref(), blk(K_PRESS_SOURCE), dep(K_PRESS_SOURCE), 1, stream));
         launch_scene_finish(dref(), 1, stream);
-      } else {
-        TIMED(K_CG_UPDATE, launch_cg_update(dref(), blk(K_CG_UPDATE), stream));
-        TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
-        launch_scene_finish(dref(), 3, stream);
       }
     }
     done += chunk;
@@ -805,7 +811,33 @@ int Batch::solver_loop(int which) {
     HIPCHK(hipMemcpyAsync(h_active_, d.n_active + which, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     if (*h_active_ <= 0) break;
-    chunk = std::min(2, maxit - done);
+    chunk = std::min(4, maxit - done);
+  }
+  return NN_OK;
+}
+
+// The Weiler CG iterations: update k (which first forms the scene sums of
+// iteration k - 1 and stops converged scenes) and matvec k, for k = 1 ..
+// limit, then update limit + 1, which only tests the last residual.  Enqueued
+// in chunks -- first the previous step's iteration count plus two, then four
+// -- with one 4-byte read of the active-scene counter per chunk; iterations
+// past convergence are near-empty launches.
+int Batch::cg_loop() {
+  int pred = 1;
+  for (auto& sp : scenes) pred = std::max(pred, sp->last_iter[2]);
+  const int maxit = std::max(1, max_iter_[2]);
+  const int last = maxit + 1;
+  int k = 1, chunk = std::min(pred + 3, last);
+  for (;;) {
+    for (int c = 0; c < chunk && k <= last; c++, k++) {
+      TIMED(K_CG_UPDATE, launch_cg_update(dref(), n_cg_blocks_, k, stream));
+      if (k <= maxit) TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
+    }
+    if (k > last) break;
+    HIPCHK(hipMemcpyAsync(h_active_, d.n_active + 2, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (*h_active_ <= 0) break;
+    chunk = 4;
   }
   return NN_OK;
 }
@@ -835,7 +867,7 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
       TIMED(K_CG_GUESS, launch_cg_guess(dref(), blk(K_CG_GUESS), stream));
       TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 1, stream));
       launch_scene_finish(dref(), 2, stream);
-      CHK(solver_loop(2));
+      CHK(cg_loop());
     }
     TIMED(K_NONPRESSURE, launch_nonpressure(dref(), blk(K_NONPRESSURE), dep(K_NONPRESSURE), stream));
     std::swap(d.vel, d.vel_alt);

@@ -77,7 +77,11 @@ struct SceneState {
   unsigned counter, max_rho_bits;
   unsigned long long pairs, pairs_fluid;
   double err_v, err_p;
-  double cg_bb, cg_rr, cg_gamma, cg_alpha, cg_beta, cg_delta, cg_theta, pad_d;
+  double cg_bb, cg_rr, cg_theta, pad_d;
+  // CG scalars gamma = r.u and alpha, kept by iteration parity: the update of
+  // iteration k reads slot (k - 1) & 1 while the scene's first workgroup
+  // writes slot k & 1 in the same launch
+  double cg_gamma2[2], cg_alpha2[2];
 };
 
 // Device pointers of a batch (passed by value to every kernel).
@@ -116,7 +120,8 @@ struct Dev {
   int* cell_count;
   int* cell_start;
   int* scan_tmp;
-  double* partials;  // 4 arrays of 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
+  double* partials;  // 8 arrays of 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
+  const int2* cg_blocks;  // CG update workgroups: (scene, first wave), up to 16 waves of one scene each
   int* n_active;     // [0] divergence, [1] pressure, [2] cg
   // Weiler CG
   float4* wx;  // iterate x
@@ -172,7 +177,7 @@ void launch_strain(const DevRef& d, int block, int depth, hipStream_t st);
 void launch_cg_init(const DevRef& d, int block, hipStream_t st);
 void launch_cg_guess(const DevRef& d, int block, hipStream_t st);
 void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStream_t st);  // depth 1, 2, 4, 8
-void launch_cg_update(const DevRef& d, int block, hipStream_t st);
+void launch_cg_update(const DevRef& d, int nblocks, int k, hipStream_t st);  // iteration k >= 1
 void launch_nonpressure(const DevRef& d, int block, int depth, hipStream_t st);
 void launch_press_source(const DevRef& d, int block, int depth, int reduce, hipStream_t st);
 void launch_press_update(const DevRef& d, int block, int depth, hipStream_t st);

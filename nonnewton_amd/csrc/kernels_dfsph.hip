@@ -118,7 +118,6 @@ __global__ void k_solver_begin(const Dev* __restrict__ dp, int which) {
       st.iter_cg = 0;
       st.cg_rr = 0.0;
       st.cg_bb = 0.0;
-      st.cg_beta = 0.0;
     }
     st.counter = 0u;
     if (act) atomicAdd(&cnt, 1);
@@ -359,19 +358,21 @@ __global__ void k_press_source(const Dev* __restrict__ dp, int reduce) {
   if (reduce) wave_partials<1>(d, (int)(i >> 6), v);
 }
 
-// Second stage of the per-scene reductions: one 256-thread workgroup per scene
+// Second stage of the per-scene reductions: one 1024-thread workgroup per scene
 // adds the partials of its waves and applies the loop test of the solver.
 //   which 0: divergence solve (:1695-1703), 1: pressure solve (:1778-1784),
 //         2: CG start (after cg_guess and the first matvec),
-//         3: CG iteration (after an update and a matvec),
 //         4: scale of the initial guess (after cg_init, SF_WARM_CG scenes).
-// Thread t adds entries t, t + 256, ... in order; the four wavefront sums are
-// combined as (W0 + W1) + (W2 + W3): a fixed order, the same bits every run.
-constexpr int FINISH_THREADS = 256;
+// (The CG's per-iteration reduction is done inside k_cg_update.)
+// Thread t adds entries t, t + 1024, ... in order (two interleaved chains);
+// the sixteen wavefront sums are combined by a fixed pairwise tree: a fixed
+// order, the same bits every run.
+constexpr int FINISH_THREADS = 1024;
+constexpr int FINISH_WAVES = FINISH_THREADS / 64;
 
 __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __restrict__ dp, int which) {
   const Dev& d = *dp;
-  __shared__ double red[4][FINISH_THREADS / 64];
+  __shared__ double red[4][FINISH_WAVES];
   const int s = blockIdx.x;
   const int t = threadIdx.x;
   const SceneDev& sc = d.scenes[s];
@@ -382,18 +383,16 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __re
     case 0: part = st->active_v != 0; break;
     case 1: part = st->active_p != 0; break;
     case 2: part = weiler; break;
-    case 4: part = weiler && (sc.flags & SF_WARM_CG); break;
-    default: part = weiler && st->active_cg != 0; break;
+    default: part = weiler && (sc.flags & SF_WARM_CG); break;  // 4
   }
   if (!part) return;  // uniform over the workgroup
   // partial arrays read and their granularity (4 = four-lane sweep, a
   // partial per 16 particles)
-  const int nr = which == 2 ? 4 : (which == 3 ? 3 : (which == 4 ? 2 : 1));
+  const int nr = which == 2 ? 4 : (which == 4 ? 2 : 1);
   double tot[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < nr; r++) {
-    // matvec partials (array 3 at the start, 2 in an iteration) and cg_init's
-    // are per 16 particles
-    const int g = which == 4 ? 4 : (which == 2 ? (r == 3 ? 4 : 1) : (which == 3 ? (r == 2 ? 4 : 1) : 1));
+    // the first matvec's partials (array 3) and cg_init's are per 16 particles
+    const int g = which == 4 ? 4 : (which == 2 ? (r == 3 ? 4 : 1) : 1);
     const double* src = d.partials + (long)r * 4 * d.W;
     const int lo = g * sc.wave_begin, hi = g * sc.wave_end;
     double a0 = 0.0, a1 = 0.0;
@@ -408,7 +407,16 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __re
   }
   __syncthreads();
   if (t != 0) return;
-  for (int r = 0; r < nr; r++) tot[r] = (red[r][0] + red[r][1]) + (red[r][2] + red[r][3]);
+  for (int r = 0; r < nr; r++) {  // fixed pairwise tree over the wave sums
+    double w[FINISH_WAVES];
+#pragma unroll
+    for (int q = 0; q < FINISH_WAVES; q++) w[q] = red[r][q];
+#pragma unroll
+    for (int span = 1; span < FINISH_WAVES; span *= 2)
+#pragma unroll
+      for (int q = 0; q + span < FINISH_WAVES; q += 2 * span) w[q] += w[q + span];
+    tot[r] = w[0];
+  }
   if (which == 0) {
     const double err = tot[0] / sc.nf_d;
     const int ran = st->iter_v + 1;
@@ -432,36 +440,17 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __re
   } else if (which == 4) {
     // tot: e.r0, e.Ae
     st->cg_theta = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
-  } else if (which == 2) {
-    // tot: b.b, r.r, r.u, u.w of the initial residual
+  } else {
+    // which 2, tot: b.b, r.r, r.u, u.w of the initial residual; the CG's
+    // iteration 1 reads alpha and gamma from parity slot 1
     st->cg_bb = tot[0];
     st->cg_rr = tot[1];
-    st->cg_gamma = tot[2];
-    st->cg_delta = tot[3];
-    st->cg_alpha = tot[2] / tot[3];
-    st->cg_beta = 0.0;
+    st->cg_gamma2[1] = tot[2];
+    st->cg_alpha2[1] = tot[2] / tot[3];
     st->iter_cg = 0;
     const int act = (tot[0] > 0.0 && tot[1] > sc.cg_tol2 * tot[0]) ? 1 : 0;
     st->active_cg = act;
     if (act) atomicAdd(&d.n_active[2], 1);
-  } else {
-    // tot: r.r, r.u (update sweep), u.w (matvec sweep)
-    const int it = st->iter_cg + 1;
-    st->iter_cg = it;
-    st->cg_rr = tot[0];
-    const bool conv = tot[0] <= sc.cg_tol2 * st->cg_bb;
-    if (conv || it >= sc.max_iter_cg) {
-      st->active_cg = 0;
-      if (!conv) st->status |= ST_NOCONV;
-      atomicSub(&d.n_active[2], 1);
-    } else {
-      const double ga = tot[1], de = tot[2];
-      const double be = ga / st->cg_gamma;
-      st->cg_alpha = ga / (de - be * ga / st->cg_alpha);
-      st->cg_beta = be;
-      st->cg_gamma = ga;
-      st->cg_delta = de;
-    }
   }
 }
 

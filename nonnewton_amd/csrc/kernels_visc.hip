@@ -16,12 +16,14 @@
 // preconditioner M, FP64 dot products, stop when ||r|| <= tol ||b|| with b = v.
 // A is symmetric (f_i s_ij = 10 dt mu_ij m F / (rho_i rho_j |r| h (|r|^2 +
 // 0.01 h^2)) with mu_ij the mean), so CG applies; it runs in the
-// Chronopoulos-Gear arrangement, which needs one reduction per iteration:
-//   update sweep:  p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
+// Chronopoulos-Gear arrangement, which needs one reduction per iteration, and
+// an iteration is two launches:
+//   update sweep:  the scene's sums of the previous iteration -> test r.r,
+//                  gamma = r.u, delta = u.w, beta = gamma / gamma_old,
+//                  alpha = gamma / (delta - beta gamma / alpha_old); then
+//                  p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
 //                  u = M r; partials (r.r, r.u)
 //   matvec sweep:  w = A u (gathers u of the neighbors); partial u.w
-//   scene finish:  gamma = r.u, delta = u.w, beta = gamma / gamma_old,
-//                  alpha = gamma / (delta - beta gamma / alpha_old), test r.r
 // The pair scalars s_ij are formed once per solve (cg_init) and stored next to
 // the neighbor lists, so a CG matvec needs no sqrt or division.  cg_init and
 // cg_matvec run four lanes per particle, each lane over every fourth list
@@ -230,9 +232,9 @@ __global__ void k_cg_guess(const Dev* __restrict__ dp) {
   wave_partials<3>(d, (int)(i >> 6), v);
 }
 
-// w = A u (u gathered from the neighbors); partial u.w into partial array
-// 3 (start: after cg_init, every Weiler scene) or 2 (an iteration, active
-// scenes).  D list positions of a lane per batch: their list words and pair
+// w = A u (u gathered from the neighbors); partial u.w into partial array 3
+// (start: after cg_guess, every Weiler scene; read by the start reduction --
+// otherwise: an iteration, active scenes; read by the next update).  D list positions of a lane per batch: their list words and pair
 // scalars, then their neighbors' positions and u, are loaded before any of
 // them is used (a shorter chain of dependent loads); the terms are added in
 // list order whatever D is.
@@ -289,43 +291,118 @@ __global__ void k_cg_matvec(const Dev* __restrict__ dp, int start) {
     d.ww[i] = make_float4(wx, wy, wz, 0.f);
     v[0] = (double)ui.x * (double)wx + (double)ui.y * (double)wy + (double)ui.z * (double)wz;
   }
-  wave_partials<1>(d, hw, v, start ? 3 : 2);
+  wave_partials<1>(d, hw, v, 3);
 }
 
-// p = u + beta p, s = w + beta s (p = u, s = w on the first iteration);
-// x += alpha p; r -= alpha s; u = M r; partials (r.r, r.u).
-__global__ void k_cg_update(const Dev* __restrict__ dp) {
+// Iteration k of the CG with the scene's reduction of iteration k - 1 done in
+// the same launch (no separate reduction kernel per iteration).  Workgroup =
+// up to 16 waves of one scene (d.cg_blocks).  Every workgroup of a scene
+// first adds the scene's partials -- r.r and r.u of update k - 1, u.w of
+// matvec k - 1 -- in one fixed order, so all of them obtain the same bits and
+// take the same decision:
+//   converged (r.r <= tol^2 b.b) or k - 1 iterations reached the limit: the
+//     scene leaves the solve (its first workgroup records it);
+//   otherwise beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma /
+//     alpha_old) (iteration 1 takes alpha from the start reduction, beta = 0)
+// and then p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
+// u = M r, with partials (r.r, r.u) into arrays 4 + 2 (k & 1), 5 + 2 (k & 1).
+constexpr int UPD_THREADS = 1024;
+constexpr int UPD_WAVES = UPD_THREADS / 64;
+
+__global__ __launch_bounds__(UPD_THREADS) void k_cg_update(const Dev* __restrict__ dp, int k) {
   const Dev& d = *dp;
-  int i, s;
-  if (!slot_scene(d, i, s)) return;
+  __shared__ double red[3][UPD_WAVES];
+  const int2 e = d.cg_blocks[blockIdx.x];
+  const int s = e.x;
   const SceneDev& sc = d.scenes[s];
-  const SceneState* st = d.state + s;
-  if (!weiler_scene(sc) || !st->active_cg) return;
-  const bool first = st->iter_cg == 0;
-  const float al = (float)st->cg_alpha;
-  const float be = (float)st->cg_beta;
+  SceneState* st = d.state + s;
+  if (!st->active_cg) return;  // uniform over the workgroup
+  const int t = threadIdx.x;
+  const bool lead = e.y == sc.wave_begin && t == 0;
+  double al, be;
+  if (k == 1) {
+    al = st->cg_alpha2[1];
+    be = 0.0;
+  } else {
+    const int q = (k - 1) & 1;
+    const double* src[3] = {d.partials + (long)(4 + 2 * q) * 4 * d.W, d.partials + (long)(5 + 2 * q) * 4 * d.W,
+                            d.partials + 3L * 4 * d.W};
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const int g = r == 2 ? 4 : 1;  // the matvec's partials are per 16 particles
+      const int lo = g * sc.wave_begin, hi = g * sc.wave_end;
+      double a0 = 0.0, a1 = 0.0;
+      int j = lo + t;
+      for (; j + UPD_THREADS < hi; j += 2 * UPD_THREADS) {
+        a0 += src[r][j];
+        a1 += src[r][j + UPD_THREADS];
+      }
+      if (j < hi) a0 += src[r][j];
+      const double w = wave_sum(a0 + a1);
+      if ((t & 63) == 0) red[r][t >> 6] = w;
+    }
+    __syncthreads();
+    double tot[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {  // fixed pairwise tree over the wave sums
+      double w[UPD_WAVES];
+#pragma unroll
+      for (int v = 0; v < UPD_WAVES; v++) w[v] = red[r][v];
+#pragma unroll
+      for (int span = 1; span < UPD_WAVES; span *= 2)
+#pragma unroll
+        for (int v = 0; v + span < UPD_WAVES; v += 2 * span) w[v] += w[v + span];
+      tot[r] = w[0];
+    }
+    const double rr = tot[0], ga = tot[1], de = tot[2];
+    const int it = k - 1;
+    const bool conv = rr <= sc.cg_tol2 * st->cg_bb;
+    if (conv || it >= sc.max_iter_cg) {
+      if (lead) {
+        st->iter_cg = it;
+        st->cg_rr = rr;
+        st->active_cg = 0;
+        if (!conv) st->status |= ST_NOCONV;
+        atomicSub(&d.n_active[2], 1);
+      }
+      return;
+    }
+    be = ga / st->cg_gamma2[q];
+    al = ga / (de - be * ga / st->cg_alpha2[q]);
+    if (lead) {
+      st->iter_cg = it;
+      st->cg_rr = rr;
+      st->cg_gamma2[k & 1] = ga;
+      st->cg_alpha2[k & 1] = al;
+    }
+  }
+  const bool first = k == 1;
+  const float alf = (float)al, bef = (float)be;
+  const int w = e.y + (t >> 6);
+  if (w >= sc.wave_end) return;  // whole waves past the scene
+  const int i = w * 64 + (t & 63);
   double v[2] = {0.0, 0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
     const float4 u = d.wu[i];
-    const float4 w = d.ww[i];
+    const float4 ww = d.ww[i];
     float4 p, sv;
     if (first) {
       p = u;
-      sv = w;
+      sv = ww;
     } else {
       const float4 po = d.wp[i];
       const float4 so = d.ws[i];
-      p = make_float4(u.x + be * po.x, u.y + be * po.y, u.z + be * po.z, 0.f);
-      sv = make_float4(w.x + be * so.x, w.y + be * so.y, w.z + be * so.z, 0.f);
+      p = make_float4(u.x + bef * po.x, u.y + bef * po.y, u.z + bef * po.z, 0.f);
+      sv = make_float4(ww.x + bef * so.x, ww.y + bef * so.y, ww.z + bef * so.z, 0.f);
     }
     float4 x = d.wx[i];
     float4 r = d.wr[i];
-    x.x += al * p.x;
-    x.y += al * p.y;
-    x.z += al * p.z;
-    r.x -= al * sv.x;
-    r.y -= al * sv.y;
-    r.z -= al * sv.z;
+    x.x += alf * p.x;
+    x.y += alf * p.y;
+    x.z += alf * p.z;
+    r.x -= alf * sv.x;
+    r.y -= alf * sv.y;
+    r.z -= alf * sv.z;
     const float4 w0 = d.wd0[i];
     const float2 w1 = d.wd1[i];
     const float zx = w0.x * r.x + w0.y * r.y + w0.z * r.z;
@@ -339,7 +416,7 @@ __global__ void k_cg_update(const Dev* __restrict__ dp) {
     v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[1] = (double)r.x * (double)zx + (double)r.y * (double)zy + (double)r.z * (double)zz;
   }
-  wave_partials<2>(d, (int)(i >> 6), v);
+  wave_partials<2>(d, w, v, 4 + 2 * (k & 1));
 }
 
 static inline dim3 grid_of(const DevRef& d, int block) { return dim3((d.h.Pn + block - 1) / block); }
@@ -369,8 +446,8 @@ void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStrea
     default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d.g, start); break;
   }
 }
-void launch_cg_update(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_update, grid_of(d, block), dim3(block), 0, st, d.g);
+void launch_cg_update(const DevRef& d, int nblocks, int k, hipStream_t st) {
+  if (nblocks > 0) hipLaunchKernelGGL(k_cg_update, dim3(nblocks), dim3(UPD_THREADS), 0, st, d.g, k);
 }
 
 }  // namespace nn

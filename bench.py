@@ -166,14 +166,16 @@ def run_batched(scs, steps, warmup, profile=True, block=0, tune=True):
         dominant = max(breakdown.items(), key=lambda kv: kv[1][0])[0]
         pool.profile(True, kernels=[dominant], stride=EVENT_STRIDE)
     lat = []
-    per_step_stats = []
+    raw = []
     t_start = time.perf_counter()
     for k in range(steps):
         t = time.perf_counter()
         pool.tick()
         lat.append(time.perf_counter() - t)
-        per_step_stats.append([pool.stats(q) for q in slots])
+        raw.append(pool.stats_array(slots))
     elapsed = time.perf_counter() - t_start
+    names = raw[0].dtype.names if raw else ()
+    per_step_stats = [[{f: a[f][k].item() for f in names} for k in range(len(slots))] for a in raw]
     times = {}
     if profile:
         ms, timed = pool.kernel_times()[dominant]

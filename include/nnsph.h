@@ -104,7 +104,9 @@ typedef struct nn_step_stats {
     double avg_density_err_p;
     double cg_rel_residual;
     double max_density;
-    int64_t num_pairs;       /* sum of neighbor counts */
+    int64_t num_pairs;       /* sum of neighbor counts of fluid and elastic particles (a step
+                              * builds no lists for static boundary particles; nn_neighbors
+                              * and the NN_FIELD_NUM_NEIGHBORS download complete them) */
     int64_t num_pairs_fluid; /* sum of neighbor counts of fluid particles */
 } nn_step_stats;
 

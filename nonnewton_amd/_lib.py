@@ -381,6 +381,12 @@ class Pool:
     def stats(self, slot: int) -> dict:
         return self._stats[slot].as_dict()
 
+    def stats_array(self, slots) -> np.ndarray:
+        """Copy of the last tick's statistics of `slots` as a numpy structured
+        array (fields of nn_step_stats), without per-scene Python objects."""
+        view = np.ctypeslib.as_array(self._stats)
+        return view[np.asarray(slots, dtype=np.int64)].copy()
+
     def retire(self, slot: int):
         check(self.lib.nn_pool_retire(self.h, int(slot)), "nn_pool_retire")
         self.sizes.pop(slot, None)

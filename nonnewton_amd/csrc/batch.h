@@ -104,7 +104,7 @@ class Batch {
   int neighbors(int k, int* nbr, int* cnt, int cap);
   int refresh_scene_table();  // after a parameter change
   int synchronize();
-  DevRef dref();  // the current Dev for a launch (device copy cached, uploaded when new)
+  DevRef dref();  // the current Dev for a launch
   long long bytes() const { return bytes_; }
   int block = 256;             // default workgroup size of the sweeps
   int kblock[K_COUNT] = {};    // per-kernel workgroup size (0 = block), set by the autotuner
@@ -127,11 +127,6 @@ class Batch {
   int scene_cap_ = 0;
   int n_cg_blocks_ = 0;  // entries of d.cg_blocks
   std::vector<void*> deferred_;
-  static constexpr int DEV_SLOTS = 16;
-  Dev* dev_slots_ = nullptr;   // device copies of recent Dev values
-  Dev slot_host_[DEV_SLOTS] = {};
-  bool slot_used_[DEV_SLOTS] = {};
-  int next_slot_ = 0;
   int scan_blocks_ = 0;
   SceneState* h_state_ = nullptr;  // pinned
   int* h_active_ = nullptr;        // pinned
@@ -142,7 +137,8 @@ class Batch {
   template <class T>
   int alloc(T*& p, size_t count);
   void release();
-  int search_impl();
+  int search_impl(int all = 1);
+  bool lists_full_ = false;
   int solver_loop(int which);
   int cg_loop();
   int build_stress_set(const std::vector<int>& which);

@@ -180,7 +180,6 @@ Batch::Batch(hipStream_t st) : stream(st) {}
 
 Batch::~Batch() {
   release();
-  if (dev_slots_) (void)hipFree(dev_slots_);
   if (h_state_) (void)hipHostFree(h_state_);
   if (h_active_) (void)hipHostFree(h_active_);
 }
@@ -208,23 +207,7 @@ void Batch::release() {
   bytes_ = 0;
 }
 
-// The device copy of the current Dev.  The batch cycles through a few Dev
-// values (the sort and the non-pressure sweep swap buffer pairs), so copies
-// are cached in DEV_SLOTS device slots and uploaded only when a value is new.
-// A slot is reused round-robin, after a stream synchronization once per lap,
-// so an upload never overwrites a slot a queued launch may still read.
-DevRef Batch::dref() {
-  for (int k = 0; k < DEV_SLOTS; k++)
-    if (slot_used_[k] && std::memcmp(&slot_host_[k], &d, sizeof(Dev)) == 0) return DevRef{d, dev_slots_ + k};
-  const int k = next_slot_;
-  next_slot_ = (next_slot_ + 1) % DEV_SLOTS;
-  if (next_slot_ == 0) (void)hipStreamSynchronize(stream);
-  std::memcpy(&slot_host_[k], &d, sizeof(Dev));
-  slot_used_[k] = true;
-  hipError_t e = hipMemcpyAsync(dev_slots_ + k, &slot_host_[k], sizeof(Dev), hipMemcpyHostToDevice, stream);
-  if (e != hipSuccess) set_error(std::string("Dev upload: ") + hipGetErrorString(e));
-  return DevRef{d, dev_slots_ + k};
-}
+DevRef Batch::dref() { return DevRef{d}; }
 
 void Batch::swap_sort_buffers() {
   std::swap(d.pos, d.pos_alt);
@@ -445,7 +428,6 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   C_cap_ = Cc;
   scene_cap_ = Sc;
   scan_blocks_ = nscan;
-  if (!dev_slots_) HIPCHK(hipMalloc((void**)&dev_slots_, sizeof(Dev) * DEV_SLOTS));
   if (!h_state_) {
     HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * 4096, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
@@ -651,14 +633,18 @@ void KernelTimer::reset() {
     CHK(timer.end(stream));           \
   } while (0)
 
-int Batch::search_impl() {
+// The counting sort and the lists.  all == 0 (inside a step) leaves static
+// boundary particles without a list; lists_full_ records which kind of search
+// ran last, so that list downloads can complete them first.
+int Batch::search_impl(int all) {
   launch_fill_i32(d.cell_count, (long)C_total_ + 1, 0, stream);
   TIMED(K_CELL_COUNT, launch_cell_count(dref(), blk(K_CELL_COUNT), stream));
   TIMED(K_SCAN, launch_scan(dref(), C_total_ + 1, stream));
   TIMED(K_BUCKET, launch_bucket(dref(), blk(K_BUCKET), stream));
   TIMED(K_PLACE, launch_place(dref(), blk(K_PLACE), stream));
   swap_sort_buffers();
-  TIMED(K_BUILD_LISTS, launch_build_lists(dref(), blk(K_BUILD_LISTS), stream));
+  TIMED(K_BUILD_LISTS, launch_build_lists(dref(), blk(K_BUILD_LISTS), all, stream));
+  lists_full_ = all != 0;
   return NN_OK;
 }
 
@@ -850,7 +836,7 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
       return NN_E_STATE;
     }
   launch_step_begin(dref(), stream);
-  CHK(search_impl());                                                  // :917
+  CHK(search_impl(0));                                                 // :917
   if (any_elastic_) TIMED(K_BVOL, launch_bvol(dref(), blk(K_BVOL), 1, stream));  // :918
   TIMED(K_CLEAR_ACC, launch_clear_acc(dref(), blk(K_CLEAR_ACC), stream));             // :920
   if (any_fluid_) {  // DFSPHSolver.substep, :1546-1555
@@ -966,6 +952,7 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
     set_error("buffer size does not match n * width * 4");
     return NN_E_ARG;
   }
+  if (field == NN_FIELD_NUM_NEIGHBORS && !lists_full_) CHK(search_impl(1));  // complete the static lists
   std::vector<int> info(n);
   HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
   float* out = (float*)host;
@@ -1108,6 +1095,7 @@ int Batch::upload_field(int k, int field, const void* host, size_t bytes) {
 }
 
 int Batch::neighbors(int k, int* nbr, int* cnt, int cap) {
+  if (!lists_full_) CHK(search_impl(1));  // a step's search skips static particles
   SceneHost& s = *scenes[k];
   const int n = s.n, pb = s.p_begin;
   if (cap < MAX_NB) {

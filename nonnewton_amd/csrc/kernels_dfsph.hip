@@ -18,8 +18,7 @@ __device__ __forceinline__ float alpha_press(const SceneDev& sc, float a) {
   return (sc.flags & SF_DIVERGENCE) ? ((a * sc.inv_dt) * sc.dt) * sc.inv_dt2 : a * sc.inv_dt2;
 }
 
-__global__ void k_step_begin(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_step_begin(Dev d) {
   for (int s = threadIdx.x; s < d.nscenes; s += blockDim.x) {
     d.state[s].status = 0;
     d.state[s].max_rho_bits = 0u;
@@ -30,8 +29,7 @@ __global__ void k_step_begin(const Dev* __restrict__ dp) {
 }
 
 // clear_accelerations_kernel, :945-951
-__global__ void k_clear_acc(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_clear_acc(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -44,8 +42,7 @@ __global__ void k_clear_acc(const Dev* __restrict__ dp) {
 // compute_density_change of the divergence solve (:1610-1645), fused: all three
 // read the same neighbors and none reads what another writes.
 template <int B>
-__global__ void k_density_alpha(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_density_alpha(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -94,8 +91,7 @@ __global__ void k_density_alpha(const Dev* __restrict__ dp) {
 }
 
 // which: 0 divergence, 1 pressure, 2 cg (cg_init activates scenes)
-__global__ void k_solver_begin(const Dev* __restrict__ dp, int which) {
-  const Dev& d = *dp;
+__global__ void k_solver_begin(Dev d, int which) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -130,8 +126,7 @@ __global__ void k_solver_begin(const Dev* __restrict__ dp, int which) {
 // alpha, writes v).  Reactions on dynamic solids are not formed: the second
 // clear_accelerations (:1552) erases them.
 template <int B>
-__global__ void k_div_update(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_div_update(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -195,8 +190,7 @@ __device__ __forceinline__ float velocity_divergence(const Dev& d, const SceneDe
 // compute_density_change (:1610-1645) + compute_density_error(0) (:1669-1675)
 // and the loop test of divergence_solve (:1695-1703)
 template <int B>
-__global__ void k_div_source(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_div_source(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -220,8 +214,7 @@ __global__ void k_div_source(const Dev* __restrict__ dp) {
 // clear_accelerations (:1552) is folded in.  v is double-buffered: the sweep
 // reads vel and writes vel_alt.
 template <int B>
-__global__ void k_nonpressure(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_nonpressure(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -339,8 +332,7 @@ __global__ void k_nonpressure(const Dev* __restrict__ dp) {
 // compute_density_adv (:1647-1667) + compute_density_error(rho0) and the loop
 // test of pressure_solve (:1778-1784)
 template <int B>
-__global__ void k_press_source(const Dev* __restrict__ dp, int reduce) {
-  const Dev& d = *dp;
+__global__ void k_press_source(Dev d, int reduce) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -370,8 +362,7 @@ __global__ void k_press_source(const Dev* __restrict__ dp, int reduce) {
 constexpr int FINISH_THREADS = 1024;
 constexpr int FINISH_WAVES = FINISH_THREADS / 64;
 
-__global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __restrict__ dp, int which) {
-  const Dev& d = *dp;
+__global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int which) {
   __shared__ double red[4][FINISH_WAVES];
   const int s = blockIdx.x;
   const int t = threadIdx.x;
@@ -458,8 +449,7 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(const Dev* __re
 // (nothing in the sweep reads v); dynamic-solid lanes gather the reaction
 // -vel_change/dt of every fluid neighbor (:1835-1841) into their acceleration.
 template <int B>
-__global__ void k_press_update(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_press_update(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -522,8 +512,7 @@ __global__ void k_press_update(const Dev* __restrict__ dp) {
 
 // advect (:1086-1093) then enforce_boundary_3D for FLUID and SOLID (:1044-1080,
 // :936-937), fused: each dynamic particle is touched once.
-__global__ void k_advect(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_advect(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -569,37 +558,37 @@ __global__ void k_advect(const Dev* __restrict__ dp) {
 static inline dim3 grid_of(const DevRef& d, int block) { return dim3((d.h.Pn + block - 1) / block); }
 
 void launch_step_begin(const DevRef& d, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, d.g);
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, d.h);
 }
 void launch_clear_acc(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d.h);
 }
 void launch_density_alpha(const DevRef& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_density_alpha, depth, grid_of(d, block), dim3(block), st, d.g);
+  DEPTH_LAUNCH(k_density_alpha, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_solver_begin(const DevRef& d, int which, hipStream_t st) {
-  hipLaunchKernelGGL(k_solver_begin, dim3(1), dim3(256), 0, st, d.g, which);
+  hipLaunchKernelGGL(k_solver_begin, dim3(1), dim3(256), 0, st, d.h, which);
 }
 void launch_div_update(const DevRef& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_div_update, depth, grid_of(d, block), dim3(block), st, d.g);
+  DEPTH_LAUNCH(k_div_update, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_div_source(const DevRef& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_div_source, depth, grid_of(d, block), dim3(block), st, d.g);
+  DEPTH_LAUNCH(k_div_source, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_nonpressure(const DevRef& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_nonpressure, depth, grid_of(d, block), dim3(block), st, d.g);
+  DEPTH_LAUNCH(k_nonpressure, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_press_source(const DevRef& d, int block, int depth, int reduce, hipStream_t st) {
-  DEPTH_LAUNCH(k_press_source, depth, grid_of(d, block), dim3(block), st, d.g, reduce);
+  DEPTH_LAUNCH(k_press_source, depth, grid_of(d, block), dim3(block), st, d.h, reduce);
 }
 void launch_press_update(const DevRef& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_press_update, depth, grid_of(d, block), dim3(block), st, d.g);
+  DEPTH_LAUNCH(k_press_update, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_advect(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d.h);
 }
 void launch_scene_finish(const DevRef& d, int which, hipStream_t st) {
-  if (d.h.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.h.nscenes), dim3(FINISH_THREADS), 0, st, d.g, which);
+  if (d.h.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.h.nscenes), dim3(FINISH_THREADS), 0, st, d.h, which);
 }
 
 }  // namespace nn

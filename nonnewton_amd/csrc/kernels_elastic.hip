@@ -13,8 +13,7 @@ namespace nn {
 
 // Elasticity.initValues_kernel, :1220-1236: x_0 and the rest volume
 // V0_i = m_i / (m_i W0 + sum_j m_j W_ij) on the lists of the first search.
-__global__ void k_elastic_init(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_elastic_init(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -43,8 +42,7 @@ __device__ __forceinline__ bool member(const Dev& d, int& q, int& s) {
 
 // computeRotations_kernel (:1249-1274) and computeStress_kernel (:1285-1323),
 // fused: the stress of particle i uses only its own rotation.
-__global__ void k_elastic_rot_stress(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_elastic_rot_stress(Dev d) {
   int q, s;
   if (!member(d, q, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -118,8 +116,7 @@ __device__ __forceinline__ void sym_mat_vec(const float* M, const float* v, floa
 }
 
 // computeForces_kernel, :1334-1375, for elastic particles: a += f / m.
-__global__ void k_elastic_force(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_elastic_force(Dev d) {
   int q, s;
   if (!member(d, q, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -179,15 +176,15 @@ __global__ void k_elastic_force(const Dev* __restrict__ dp) {
 }
 
 void launch_elastic_init(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_elastic_init, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_elastic_init, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_elastic_rot_stress(const DevRef& d, int block, hipStream_t st) {
   if (d.h.NSn <= 0) return;
-  hipLaunchKernelGGL(k_elastic_rot_stress, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_elastic_rot_stress, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_elastic_force(const DevRef& d, int block, hipStream_t st) {
   if (d.h.NSn <= 0) return;
-  hipLaunchKernelGGL(k_elastic_force, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_elastic_force, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 
 }  // namespace nn

@@ -140,8 +140,7 @@ __device__ __forceinline__ bool cell_of(const SceneDev& sc, float4 p, int& cx, i
 // the first lane of each run adds the run length with one atomic and the run's
 // lanes take consecutive ranks from it.  (Ranks only need to be unique; the
 // final order inside a cell is fixed by k_place.)
-__global__ void k_cell_count(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_cell_count(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;  // uniform over the wavefront
   const SceneDev& sc = d.scenes[s];
@@ -175,8 +174,7 @@ __global__ void k_cell_count(const Dev* __restrict__ dp) {
   d.rank[i] = r;
 }
 
-__global__ void k_bucket(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_bucket(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -184,8 +182,7 @@ __global__ void k_bucket(const Dev* __restrict__ dp) {
   d.bucket[d.cell_start[d.key[i]] + d.rank[i]] = i;
 }
 
-__global__ void k_place(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_place(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -204,16 +201,18 @@ __global__ void k_place(const Dev* __restrict__ dp) {
   d.inv[sc.p_begin + my] = dest - sc.p_begin;
 }
 
-// store_neighbors, SPH/main.py:836-847, over the sorted particles
-__global__ void k_build_lists(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+// store_neighbors, SPH/main.py:836-847, over the sorted particles.  With
+// all == 0 (the search inside a step) static boundary particles get no list:
+// after the boundary volumes of the first search no sweep reads one.
+__global__ void k_build_lists(Dev d, int all) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
   int cnt = 0;
   int cx, cy, cz;
   const float4 xi = d.pos[i];
-  if (i < sc.p_begin + sc.n && cell_of(sc, xi, cx, cy, cz)) {
+  const bool want = i < sc.p_begin + sc.n && (all || kind_of(d.info[i]) != KIND_STATIC);
+  if (want && cell_of(sc, xi, cx, cy, cz)) {
     const float thr2 = sc.thr2;
     const int P = d.P;
     const int z0 = cz > 0 ? cz - 1 : 0;
@@ -269,8 +268,7 @@ __global__ void k_build_lists(const Dev* __restrict__ dp) {
 
 // compute_static/moving_boundary_volume, SPH/main.py:1010-1035:
 // V_b = 3 / (W0 + sum over solid neighbors of W); stored as -V (solid sign).
-__global__ void k_bvol(const Dev* __restrict__ dp, int dynamic) {
-  const Dev& d = *dp;
+__global__ void k_bvol(Dev d, int dynamic) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -298,19 +296,19 @@ __global__ void k_bvol(const Dev* __restrict__ dp, int dynamic) {
 }
 
 void launch_cell_count(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_count, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_cell_count, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_bucket(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_bucket, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_place(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_place, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_place, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
-void launch_build_lists(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_lists, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g);
+void launch_build_lists(const DevRef& d, int block, int all, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_lists, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h, all);
 }
 void launch_bvol(const DevRef& d, int block, int dynamic, hipStream_t st) {
-  hipLaunchKernelGGL(k_bvol, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.g, dynamic);
+  hipLaunchKernelGGL(k_bvol, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h, dynamic);
 }
 
 }  // namespace nn

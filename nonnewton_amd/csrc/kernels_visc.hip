@@ -39,8 +39,7 @@ __device__ __forceinline__ bool weiler_scene(const SceneDev& sc) {
 // NonNewton.calcStrainRate (:461-492, with the accumulation the reference
 // omits) + the viscosity model; mu for the implicit solver.
 template <int B>
-__global__ void k_strain(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_strain(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -90,8 +89,7 @@ __global__ void k_strain(const Dev* __restrict__ dp) {
 // solve's e = x - v (into ww) and partial sums of (e.r0, e.Ae), from which the
 // scene finish forms theta = (e.r0) / (e.Ae); k_cg_guess then starts CG from
 // x0 = b + theta e, r = r0 - theta A e.
-__global__ void k_cg_init(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_cg_init(Dev d) {
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
@@ -199,8 +197,7 @@ __global__ void k_cg_init(const Dev* __restrict__ dp) {
 
 // x0 = b + theta e, r = r0 - theta A e (x0 = b, r = r0 without SF_WARM_CG),
 // u = M r; partials (b.b, r.r, r.u).
-__global__ void k_cg_guess(const Dev* __restrict__ dp) {
-  const Dev& d = *dp;
+__global__ void k_cg_guess(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
@@ -239,8 +236,7 @@ __global__ void k_cg_guess(const Dev* __restrict__ dp) {
 // them is used (a shorter chain of dependent loads); the terms are added in
 // list order whatever D is.
 template <int D>
-__global__ void k_cg_matvec(const Dev* __restrict__ dp, int start) {
-  const Dev& d = *dp;
+__global__ void k_cg_matvec(Dev d, int start) {
   int i, q, s, hw;
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
@@ -309,8 +305,7 @@ __global__ void k_cg_matvec(const Dev* __restrict__ dp, int start) {
 constexpr int UPD_THREADS = 1024;
 constexpr int UPD_WAVES = UPD_THREADS / 64;
 
-__global__ __launch_bounds__(UPD_THREADS) void k_cg_update(const Dev* __restrict__ dp, int k) {
-  const Dev& d = *dp;
+__global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
   __shared__ double red[3][UPD_WAVES];
   const int2 e = d.cg_blocks[blockIdx.x];
   const int s = e.x;
@@ -429,25 +424,25 @@ static inline dim3 grid4_of(const DevRef& d, int block) {
 }
 
 void launch_strain(const DevRef& d, int block, int depth, hipStream_t st) {
-  DEPTH_LAUNCH(k_strain, depth, grid_of(d, block), dim3(block), st, d.g);
+  DEPTH_LAUNCH(k_strain, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_cg_init(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d.g);
+  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d.h);
 }
 void launch_cg_guess(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d.g);
+  hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d.h);
 }
 void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStream_t st) {
   const dim3 g = grid4_of(d, block), b = dim3(quad_block(block));
   switch (depth) {
-    case 1: hipLaunchKernelGGL(k_cg_matvec<1>, g, b, 0, st, d.g, start); break;
-    case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d.g, start); break;
-    case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d.g, start); break;
-    default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d.g, start); break;
+    case 1: hipLaunchKernelGGL(k_cg_matvec<1>, g, b, 0, st, d.h, start); break;
+    case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d.h, start); break;
+    case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d.h, start); break;
+    default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d.h, start); break;
   }
 }
 void launch_cg_update(const DevRef& d, int nblocks, int k, hipStream_t st) {
-  if (nblocks > 0) hipLaunchKernelGGL(k_cg_update, dim3(nblocks), dim3(UPD_THREADS), 0, st, d.g, k);
+  if (nblocks > 0) hipLaunchKernelGGL(k_cg_update, dim3(nblocks), dim3(UPD_THREADS), 0, st, d.h, k);
 }
 
 }  // namespace nn

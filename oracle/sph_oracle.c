@@ -1311,8 +1311,11 @@ int32_t or_step(or_sim *s, or_stats *st)
     }
     advect(s);
     clamp_boundary(s);
+    /* pairs of the lists a step uses: fluid and elastic particles (the static
+     * boundary's lists serve only the boundary volumes of the first search) */
     long long np = 0;
-    for (int i = 0; i < s->n; i++) np += s->nnbr[i];
+    for (int i = 0; i < s->n; i++)
+        if (s->kind[i] != OR_KIND_STATIC) np += s->nnbr[i];
     st->num_pairs = np;
     st->status = s->status;
     return s->status;

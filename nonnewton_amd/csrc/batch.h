@@ -138,7 +138,11 @@ class Batch {
   int alloc(T*& p, size_t count);
   void release();
   int search_impl(int all = 1);
-  bool lists_full_ = false;
+  int complete_lists();
+  // neighbor lists: 0 none valid (new layout), 1 built by a step's search
+  // (static particles without lists; that search's positions in pos_alt),
+  // 2 complete
+  int lists_state_ = 0;
   int solver_loop(int which);
   int cg_loop();
   int build_stress_set(const std::vector<int>& which);

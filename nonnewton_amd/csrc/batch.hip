@@ -464,6 +464,7 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
     for (auto& al : old_allocs) (void)hipFree(al.p);
   }
   d = nd;
+  lists_state_ = 0;
   return NN_OK;
 }
 
@@ -522,6 +523,7 @@ int Batch::append_in_place(const std::vector<std::shared_ptr<SceneHost>>& member
   for (size_t k = nk; k < members.size(); k++)
     if (src[k].b) CHK(copy_scene_state(d, *members[k], src[k].b, src[k].pb, src[k].sb, src[k].ns));
   HIPCHK(hipStreamSynchronize(stream));
+  if (nk < members.size()) lists_state_ = 0;  // the new scenes have no lists here yet
   return NN_OK;
 }
 
@@ -634,7 +636,7 @@ void KernelTimer::reset() {
   } while (0)
 
 // The counting sort and the lists.  all == 0 (inside a step) leaves static
-// boundary particles without a list; lists_full_ records which kind of search
+// boundary particles without a list; lists_state_ records which kind of search
 // ran last, so that list downloads can complete them first.
 int Batch::search_impl(int all) {
   launch_fill_i32(d.cell_count, (long)C_total_ + 1, 0, stream);
@@ -644,7 +646,7 @@ int Batch::search_impl(int all) {
   TIMED(K_PLACE, launch_place(dref(), blk(K_PLACE), stream));
   swap_sort_buffers();
   TIMED(K_BUILD_LISTS, launch_build_lists(dref(), blk(K_BUILD_LISTS), all, stream));
-  lists_full_ = all != 0;
+  lists_state_ = all ? 2 : 1;
   return NN_OK;
 }
 
@@ -866,6 +868,7 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
     TIMED(K_ELASTIC_FORCE, launch_elastic_force(dref(), blk(K_ELASTIC_FORCE), stream));
   }
   TIMED(K_ADVECT, launch_advect(dref(), blk(K_ADVECT), stream));  // :933, :936-937
+  std::swap(d.pos, d.pos_alt);  // pos_alt keeps the positions of this step's search
   HIPCHK(hipGetLastError());
   CHK(gather_state());
   CHK(timer.collect());
@@ -952,7 +955,7 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
     set_error("buffer size does not match n * width * 4");
     return NN_E_ARG;
   }
-  if (field == NN_FIELD_NUM_NEIGHBORS && !lists_full_) CHK(search_impl(1));  // complete the static lists
+  if (field == NN_FIELD_NUM_NEIGHBORS) CHK(complete_lists());
   std::vector<int> info(n);
   HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
   float* out = (float*)host;
@@ -1094,8 +1097,22 @@ int Batch::upload_field(int k, int field, const void* host, size_t bytes) {
   return NN_OK;
 }
 
+// After a step, the static particles' lists are built on demand, from the
+// positions and cell order of that step's search (kept in pos_alt): the lists
+// the reference holds after a step.
+int Batch::complete_lists() {
+  if (lists_state_ == 2) return NN_OK;
+  if (lists_state_ == 0) return search_impl(1);  // laid out since: search the current positions
+  Dev at_search = d;
+  at_search.pos = d.pos_alt;
+  launch_build_lists(DevRef{at_search}, blk(K_BUILD_LISTS), 1, stream);
+  HIPCHK(hipGetLastError());
+  lists_state_ = 2;
+  return NN_OK;
+}
+
 int Batch::neighbors(int k, int* nbr, int* cnt, int cap) {
-  if (!lists_full_) CHK(search_impl(1));  // a step's search skips static particles
+  CHK(complete_lists());
   SceneHost& s = *scenes[k];
   const int n = s.n, pb = s.p_begin;
   if (cap < MAX_NB) {

@@ -512,14 +512,19 @@ __global__ void k_press_update(Dev d) {
 
 // advect (:1086-1093) then enforce_boundary_3D for FLUID and SOLID (:1044-1080,
 // :936-937), fused: each dynamic particle is touched once.
+// New positions go to pos_alt (the host swaps the pair afterwards), so the
+// positions the step's search used stay available until the next search.
 __global__ void k_advect(Dev d) {
   int i, s;
   if (!slot_scene(d, i, s)) return;
   const SceneDev& sc = d.scenes[s];
   if (i >= sc.p_begin + sc.n) return;
   const int kind = kind_of(d.info[i]);
-  if (kind == KIND_STATIC) return;
   float4 x = d.pos[i];
+  if (kind == KIND_STATIC) {
+    d.pos_alt[i] = x;
+    return;
+  }
   float4 v = d.vel[i];
   if (kind == KIND_ELASTIC) {
     const float4 a = d.acc[i];
@@ -551,7 +556,7 @@ __global__ void k_advect(Dev d) {
   }
   if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z)))
     atomicOr(&d.state[s].status, ST_NAN);
-  d.pos[i] = x;
+  d.pos_alt[i] = x;
   d.vel[i] = v;
 }
 

@@ -392,6 +392,8 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   int2* cgb;
   A(cgb, (size_t)nd.W / 16 + Sc + 1);
   nd.cg_blocks = cgb;
+  A(nd.cg_list, 2 * (size_t)nd.W);
+  A(nd.cg_list_n, 2);
   A(nd.n_active, 4);
   A(nd.wx, Pc);
   A(nd.wr, Pc);
@@ -812,14 +814,39 @@ int Batch::solver_loop(int which) {
 // past convergence are near-empty launches.
 int Batch::cg_loop() {
   int pred = 1;
-  for (auto& sp : scenes) pred = std::max(pred, sp->last_iter[2]);
+  long total_waves = 0;
+  for (auto& sp : scenes) {
+    pred = std::max(pred, sp->last_iter[2]);
+    const int f = sp->dev.flags;
+    if ((f & SF_HAS_FLUID) && (f & SF_VISCOSITY) && (f & SF_WEILER)) total_waves += sp->dev.wave_end - sp->dev.wave_begin;
+  }
+  // waves expected to still iterate at iteration k, from each scene's count
+  // in the previous step (plus a margin): while they are many the matvec runs
+  // on the full grid, once they are few on a grid sized to them (the list of
+  // waves the update writes holds the exact set either way)
+  auto expected_waves = [&](int k) {
+    long w = 0;
+    for (auto& sp : scenes) {
+      const int f = sp->dev.flags;
+      if ((f & SF_HAS_FLUID) && (f & SF_VISCOSITY) && (f & SF_WEILER) && sp->last_iter[2] + 2 >= k)
+        w += sp->dev.wave_end - sp->dev.wave_begin;
+    }
+    return w;
+  };
   const int maxit = std::max(1, max_iter_[2]);
   const int last = maxit + 1;
   int k = 1, chunk = std::min(pred + 3, last);
   for (;;) {
     for (int c = 0; c < chunk && k <= last; c++, k++) {
       TIMED(K_CG_UPDATE, launch_cg_update(dref(), n_cg_blocks_, k, stream));
-      if (k <= maxit) TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
+      if (k > maxit) continue;
+      const long ew = expected_waves(k);
+      if (4 * ew >= total_waves) {
+        TIMED(K_CG_MATVEC, launch_cg_matvec(dref(), blk(K_CG_MATVEC), dep(K_CG_MATVEC), 0, stream));
+      } else {
+        const int grid = (int)std::min<long>(std::max<long>(ew, 64), 2048);
+        TIMED(K_CG_MATVEC, launch_cg_matvec_list(dref(), dep(K_CG_MATVEC), k, grid, stream));
+      }
     }
     if (k > last) break;
     HIPCHK(hipMemcpyAsync(h_active_, d.n_active + 2, sizeof(int), hipMemcpyDeviceToHost, stream));

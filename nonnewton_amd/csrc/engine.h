@@ -122,6 +122,8 @@ struct Dev {
   int* scan_tmp;
   double* partials;  // 8 arrays of 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
   const int2* cg_blocks;  // CG update workgroups: (scene, first wave), up to 16 waves of one scene each
+  int* cg_list;           // 2 x W: waves of the scenes still iterating, by iteration parity
+  int* cg_list_n;         // 2 counts
   int* n_active;     // [0] divergence, [1] pressure, [2] cg
   // Weiler CG
   float4* wx;  // iterate x
@@ -176,6 +178,7 @@ void launch_strain(const DevRef& d, int block, int depth, hipStream_t st);
 void launch_cg_init(const DevRef& d, int block, hipStream_t st);
 void launch_cg_guess(const DevRef& d, int block, hipStream_t st);
 void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStream_t st);  // depth 1, 2, 4, 8
+void launch_cg_matvec_list(const DevRef& d, int depth, int k, int grid, hipStream_t st);  // iteration k
 void launch_cg_update(const DevRef& d, int nblocks, int k, hipStream_t st);  // iteration k >= 1
 void launch_nonpressure(const DevRef& d, int block, int depth, hipStream_t st);
 void launch_press_source(const DevRef& d, int block, int depth, int reduce, hipStream_t st);

@@ -114,6 +114,7 @@ __global__ void k_solver_begin(Dev d, int which) {
       st.iter_cg = 0;
       st.cg_rr = 0.0;
       st.cg_bb = 0.0;
+      if (s == 0) d.cg_list_n[0] = d.cg_list_n[1] = 0;
     }
     st.counter = 0u;
     if (act) atomicAdd(&cnt, 1);

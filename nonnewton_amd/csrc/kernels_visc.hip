@@ -290,6 +290,72 @@ __global__ void k_cg_matvec(Dev d, int start) {
   wave_partials<1>(d, hw, v, 3);
 }
 
+// Matvec k of the CG over the waves listed by update k (the scenes still
+// iterating), for the late iterations when few scenes are left: a grid sized
+// to the expected work, workgroup = 256 threads = the four lanes of one wave's
+// 64 particles, looping over the list.  Same arithmetic and the same partials
+// (one per 16 particles, array 3) as k_cg_matvec.
+constexpr int MVL_THREADS = 256;
+
+template <int D>
+__global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
+  const int count = d.cg_list_n[k & 1];
+  const int* list = d.cg_list + (long)(k & 1) * d.W;
+  const int q = threadIdx.x & 3;
+  for (int u = blockIdx.x; u < count; u += gridDim.x) {
+    const int w = list[u];
+    const int s = uniform(d.wave_scene[w]);
+    const SceneDev& sc = d.scenes[s];
+    const int i = w * 64 + (threadIdx.x >> 2);
+    const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
+      const float4 xi = d.pos[i];
+      ui = d.wu[i];
+      const int cnt = d.nnbr[i];
+      const long P = d.P;
+      for (int kk = q; kk < cnt; kk += 4 * D) {
+        int j[D];
+        float sp[D];
+        float4 xj[D], uj[D];
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          const bool in = kk + 4 * e < cnt;
+          j[e] = in ? d.nbr[(kk + 4 * e) * P + i] : i;
+          sp[e] = in ? d.spair[(kk + 4 * e) * P + i] : 0.0f;
+        }
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          xj[e] = d.pos[j[e]];
+          uj[e] = d.wu[j[e]];
+        }
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          if (kk + 4 * e < cnt && xj[e].w > 0.0f) {
+            const float rx = xi.x - xj[e].x, ry = xi.y - xj[e].y, rz = xi.z - xj[e].z;
+            const float t = sp[e] * ((ui.x - uj[e].x) * rx + (ui.y - uj[e].y) * ry + (ui.z - uj[e].z) * rz);
+            ax += t * rx;
+            ay += t * ry;
+            az += t * rz;
+          }
+        }
+      }
+    }
+    ax = quad_sum(ax);
+    ay = quad_sum(ay);
+    az = quad_sum(az);
+    double v[1] = {0.0};
+    if (valid && q == 0) {
+      const float f = sc.dt / d.rho[i];
+      const float wx = ui.x - f * ax, wy = ui.y - f * ay, wz = ui.z - f * az;
+      d.ww[i] = make_float4(wx, wy, wz, 0.f);
+      v[0] = (double)ui.x * (double)wx + (double)ui.y * (double)wy + (double)ui.z * (double)wz;
+    }
+    wave_partials<1>(d, 4 * w + (threadIdx.x >> 6), v, 3);  // the quarter-wave index k_cg_matvec uses
+  }
+}
+
 // Iteration k of the CG with the scene's reduction of iteration k - 1 done in
 // the same launch (no separate reduction kernel per iteration).  Workgroup =
 // up to 16 waves of one scene (d.cg_blocks).  Every workgroup of a scene
@@ -311,8 +377,9 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
   const int s = e.x;
   const SceneDev& sc = d.scenes[s];
   SceneState* st = d.state + s;
-  if (!st->active_cg) return;  // uniform over the workgroup
   const int t = threadIdx.x;
+  if (blockIdx.x == 0 && t == 0) d.cg_list_n[(k + 1) & 1] = 0;  // for update k + 1
+  if (!st->active_cg) return;  // uniform over the workgroup
   const bool lead = e.y == sc.wave_begin && t == 0;
   double al, be;
   if (k == 1) {
@@ -370,6 +437,15 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
       st->cg_gamma2[k & 1] = ga;
       st->cg_alpha2[k & 1] = al;
     }
+  }
+  // the scene iterates on: its first workgroup lists its waves for matvec k
+  if (e.y == sc.wave_begin) {
+    __shared__ int base;
+    const int nw = sc.wave_end - sc.wave_begin;
+    if (t == 0) base = atomicAdd(&d.cg_list_n[k & 1], nw);
+    __syncthreads();
+    int* out = d.cg_list + (long)(k & 1) * d.W + base;
+    for (int m = t; m < nw; m += UPD_THREADS) out[m] = sc.wave_begin + m;
   }
   const bool first = k == 1;
   const float alf = (float)al, bef = (float)be;
@@ -439,6 +515,15 @@ void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStrea
     case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d.h, start); break;
     case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d.h, start); break;
     default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d.h, start); break;
+  }
+}
+void launch_cg_matvec_list(const DevRef& d, int depth, int k, int grid, hipStream_t st) {
+  const dim3 g = dim3(grid > 0 ? grid : 1), b = dim3(MVL_THREADS);
+  switch (depth) {
+    case 1: hipLaunchKernelGGL(k_cg_matvec_list<1>, g, b, 0, st, d.h, k); break;
+    case 4: hipLaunchKernelGGL(k_cg_matvec_list<4>, g, b, 0, st, d.h, k); break;
+    case 8: hipLaunchKernelGGL(k_cg_matvec_list<8>, g, b, 0, st, d.h, k); break;
+    default: hipLaunchKernelGGL(k_cg_matvec_list<2>, g, b, 0, st, d.h, k); break;
   }
 }
 void launch_cg_update(const DevRef& d, int nblocks, int k, hipStream_t st) {

@@ -155,3 +155,26 @@ def test_sampled_kernel_timing():
     assert all(n == 0 for k, (_, n) in times.items() if k != "cg_matvec")
     pool.profile(False)
     pool.close()
+
+
+def test_late_cg_iterations_on_the_wave_list():
+    """An easy scene and a very viscous one in one pool: once the easy scene's CG
+    has stopped, the late iterations run on the list of the hard scene's waves
+    (a grid sized to them).  Both scenes equal themselves run alone, bit for bit,
+    with the same iteration counts."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    easy = scenes.mixed_scene(rng, 3000, 0, 0.2, 1.0, False, "easy")
+    hard = scenes.mixed_scene(rng, 6000, 2, 90.0, 1.0, False, "hard")
+    pool = L.Pool(8 << 30)
+    qe = pool.admit(easy.params, easy.x, easy.kind, easy.v)
+    qh = pool.admit(hard.params, hard.x, hard.kind, hard.v)
+    for _ in range(5):
+        pool.tick()
+    se, sh = pool.stats(qe), pool.stats(qh)
+    assert sh["iterations_cg"] > se["iterations_cg"] + 4
+    for sc, slot in ((easy, qe), (hard, qh)):
+        ref, st = alone(sc, 5)
+        for f, a in ref.items():
+            assert np.array_equal(pool.read(slot, f), a), (sc.name, f)
+        assert pool.stats(slot)["iterations_cg"] == st["iterations_cg"]
+    pool.close()

@@ -31,6 +31,34 @@ int hip_fail(hipError_t e, const char* what) {
 
 static inline int round_up64(int v) { return (v + 63) & ~63; }
 
+// The neighbor lists of slots [pb, pb + n) into host memory (n * MAX_NB ints),
+// in the device's grouped layout restricted to those slots: position m of
+// slot pb + t is at host_nb(n, t, m).
+static hipError_t copy_lists(int* host, const Dev& d, int pb, int n, hipStream_t st) {
+  return hipMemcpy2DAsync(host, sizeof(int4) * n, d.nbr + 4L * pb, sizeof(int4) * d.P, sizeof(int4) * n, MAX_NB / 4,
+                          hipMemcpyDeviceToHost, st);
+}
+static inline size_t host_nb(int n, int t, int m) { return 4 * ((size_t)(m >> 2) * n + t) + (m & 3); }
+// After the copy has completed: list entries to scene-local slots, each
+// checked to lie in [0, n).
+static int lists_to_local(std::vector<int>& nb, const std::vector<int>& cnt, int pb, int n) {
+  for (int t = 0; t < n; t++) {
+    if (cnt[t] < 0 || cnt[t] > MAX_NB) {
+      set_error("neighbor count out of range");
+      return NN_E_STATE;
+    }
+    for (int m = 0; m < cnt[t]; m++) {
+      int& j = nb[host_nb(n, t, m)];
+      j -= pb;
+      if (j < 0 || j >= n) {
+        set_error("neighbor list entry outside its scene");
+        return NN_E_STATE;
+      }
+    }
+  }
+  return NN_OK;
+}
+
 // cubic_kernel (SPH/main.py:859-880), host copy of the device function
 static float host_kern_w(float h, float kW, float rn) {
   float res = 0.0f;
@@ -694,16 +722,16 @@ int Batch::build_stress_set(const std::vector<int>& which) {
     std::vector<int> info(n), cnt(n), nb((size_t)n * MAX_NB);
     HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(cnt.data(), d.nnbr + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpy2DAsync(nb.data(), sizeof(int) * n, d.nbr + pb, sizeof(int) * d.P, sizeof(int) * n, MAX_NB,
-                            hipMemcpyDeviceToHost, stream));
+    HIPCHK(copy_lists(nb.data(), d, pb, n, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    CHK(lists_to_local(nb, cnt, pb, n));
     std::vector<int> slot_of_id(n, -1);
     for (int t = 0; t < n; t++) slot_of_id[info[t] & 0x0FFFFFFF] = t;
     std::vector<char> mark(n, 0);
     for (int t = 0; t < n; t++) {
       if ((info[t] >> 28) != KIND_ELASTIC) continue;
       mark[info[t] & 0x0FFFFFFF] = 1;
-      for (int m = 0; m < cnt[t]; m++) mark[info[nb[(size_t)m * n + t] - pb] & 0x0FFFFFFF] = 1;
+      for (int m = 0; m < cnt[t]; m++) mark[info[nb[host_nb(n, t, m)]] & 0x0FFFFFFF] = 1;
     }
     std::vector<int> s_of(n, -1);
     for (int id = 0; id < n; id++)
@@ -723,7 +751,7 @@ int Batch::build_stress_set(const std::vector<int>& which) {
         c = MAX_REST;
       }
       s.rest_cnt[q] = c;
-      for (int m = 0; m < c; m++) s.rest[(size_t)q * MAX_REST + m] = info[nb[(size_t)m * n + t] - pb] & 0x0FFFFFFF;
+      for (int m = 0; m < c; m++) s.rest[(size_t)q * MAX_REST + m] = info[nb[host_nb(n, t, m)]] & 0x0FFFFFFF;
     }
     if (overflow) {
       set_error("an elastic particle has more than 50 rest neighbors");
@@ -1149,14 +1177,14 @@ int Batch::neighbors(int k, int* nbr, int* cnt, int cap) {
   std::vector<int> info(n), c(n), nb((size_t)n * MAX_NB);
   HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(c.data(), d.nnbr + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipMemcpy2DAsync(nb.data(), sizeof(int) * n, d.nbr + pb, sizeof(int) * d.P, sizeof(int) * n, MAX_NB,
-                          hipMemcpyDeviceToHost, stream));
+  HIPCHK(copy_lists(nb.data(), d, pb, n, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  CHK(lists_to_local(nb, c, pb, n));
   for (size_t t = 0; t < (size_t)n * cap; t++) nbr[t] = -1;
   for (int t = 0; t < n; t++) {
     const int id = info[t] & 0x0FFFFFFF;
     cnt[id] = c[t];
-    for (int m = 0; m < c[t]; m++) nbr[(size_t)id * cap + m] = info[nb[(size_t)m * n + t] - pb] & 0x0FFFFFFF;
+    for (int m = 0; m < c[t]; m++) nbr[(size_t)id * cap + m] = info[nb[host_nb(n, t, m)]] & 0x0FFFFFFF;
   }
   return NN_OK;
 }

@@ -66,21 +66,35 @@ __device__ __forceinline__ float3 kern_grad(float h, float kG, float rx, float r
   return g;
 }
 
-// Walk the neighbor list of particle i in list order, B entries at a time: the
-// B list words are loaded together, then the B neighbors' data (load(j)), and
-// only then body(j, data) runs for each in order.  The loads of a batch are
-// independent, so a wavefront has B gathers in flight instead of one; the
-// order of the per-neighbor terms (and so every sum) is unchanged.  Lanes past
-// the end of their list load their own particle's data and skip the body.
+// Word of list position k of particle i in nbr (and of its pair scalar in
+// spair): groups of four positions, one int4 per particle and group.
+__device__ __forceinline__ long nb_word(long P, int i, int k) { return 4 * ((long)(k >> 2) * P + i) + (k & 3); }
+
+// Walk the neighbor list of particle i in list order, B entries at a time
+// (B a multiple of 4): the B list words are loaded together (B / 4 int4
+// groups), then the B neighbors' data (load(j)), and only then body(j, data)
+// runs for each in order.  The loads of a batch are independent, so a
+// wavefront has B gathers in flight instead of one; the order of the
+// per-neighbor terms (and so every sum) is unchanged.  Lanes past the end of
+// their list load their own particle's data and skip the body.
 template <int B, typename Load, typename Body>
 __device__ __forceinline__ void for_neighbors(const Dev& d, int i, int cnt, Load load, Body body) {
+  static_assert(B % 4 == 0, "whole groups of four list positions");
   using T = decltype(load(0));
   const long P = d.P;
+  const int4* groups = reinterpret_cast<const int4*>(d.nbr);
   for (int k0 = 0; k0 < cnt; k0 += B) {
     int j[B];
     T v[B];
 #pragma unroll
-    for (int u = 0; u < B; u++) j[u] = (k0 + u < cnt) ? d.nbr[(long)(k0 + u) * P + i] : i;
+    for (int g = 0; g < B / 4; g++) {
+      const int k = k0 + 4 * g;
+      const int4 w = k < cnt ? groups[(long)(k >> 2) * P + i] : make_int4(i, i, i, i);
+      j[4 * g + 0] = w.x;
+      j[4 * g + 1] = k + 1 < cnt ? w.y : i;
+      j[4 * g + 2] = k + 2 < cnt ? w.z : i;
+      j[4 * g + 3] = k + 3 < cnt ? w.w : i;
+    }
 #pragma unroll
     for (int u = 0; u < B; u++) v[u] = load(j[u]);
 #pragma unroll

@@ -9,8 +9,13 @@
 //    16-byte gather per neighbor yields position, volume and material.
 //  * vel = float4 (vx, vy, vz, 0); acc = float4.
 //  * info = kind << 28 | original index (scene-local); -1 on padding slots.
-//  * Neighbor lists: nbr[k * P + i], k < 60, so lane i and lane i+1 read adjacent
-//    words; nnbr[i] holds the count.
+//  * Neighbor lists in groups of four positions: position k < 60 of particle i
+//    is word 4 ((k / 4) P + i) + k % 4 of nbr (nb_word), so a group is one
+//    16-byte int4 and particles i, i+1 have adjacent groups: a lane reading
+//    its list 4 positions per load, and the four lanes of a particle in the
+//    four-lane sweeps (lane q takes positions q, q+4, ...), read contiguous
+//    memory, and the list builder writes whole groups.  nnbr[i] holds the
+//    count.
 //  * Original-index ("id") space: inv[p_begin + id] = local slot; x0 (rest
 //    position, w = rest volume) and the stress-set map live there.
 //  * Cells: scene s owns global cell keys [cell_begin, cell_begin + gn0*gn1*gn2],
@@ -134,7 +139,7 @@ struct Dev {
   float4* ws;  // s = A p
   float4* wd0;  // inverse diagonal block: (d00, d01, d02, d11)
   float2* wd1;  //                          (d12, d22)
-  float* spair; // per-pair scalar s_ij of the Weiler operator, spair[k * P + i]
+  float* spair; // per-pair scalar s_ij of the Weiler operator, at the list word of the pair (nb_word)
   float4* vdiff; // x - v of the previous Weiler solve (initial guess; moves with the particle)
   float* mu;    // dynamic viscosity per particle
   float* gam;   // strain-rate norm

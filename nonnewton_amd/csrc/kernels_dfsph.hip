@@ -309,7 +309,7 @@ __global__ void k_nonpressure(Dev d) {
       const float Vi = -xi.w;
       const int cnt = d.nnbr[i];
       for (int k = 0; k < cnt; k++) {
-        const int f = d.nbr[(long)k * P + i];
+        const int f = d.nbr[nb_word(P, i, k)];
         const float4 xf = d.pos[f];
         if (!(xf.w > 0.0f)) continue;
         const float4 vf = d.vel[f];
@@ -494,7 +494,7 @@ __global__ void k_press_update(Dev d) {
     const float Vi = -xi.w;
     float4 a = d.acc[i];
     for (int k = 0; k < cnt; k++) {
-      const int f = d.nbr[(long)k * P + i];
+      const int f = d.nbr[nb_word(P, i, k)];
       const float4 xf = d.pos[f];
       if (!(xf.w > 0.0f)) continue;
       const float kf = (d.src[f] - 1.0f) * alpha_press(sc, d.alpha[f]);

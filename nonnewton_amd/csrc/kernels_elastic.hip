@@ -23,7 +23,7 @@ __global__ void k_elastic_init(Dev d) {
   const int P = d.P;
   float den = sc.mass * sc.W0;
   for (int k = 0; k < cnt; k++) {
-    const int j = d.nbr[(long)k * P + i];
+    const int j = d.nbr[nb_word(P, i, k)];
     const float4 xj = d.pos[j];
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     den += sc.mass * kern_w(sc.h, sc.kW, norm3(rx, ry, rz));

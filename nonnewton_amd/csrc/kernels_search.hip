@@ -212,9 +212,14 @@ __global__ void k_build_lists(Dev d, int all) {
   int cx, cy, cz;
   const float4 xi = d.pos[i];
   const bool want = i < sc.p_begin + sc.n && (all || kind_of(d.info[i]) != KIND_STATIC);
+  // list positions are collected in groups of four and each group is written
+  // as one int4 when complete (the last one possibly partial, its unused words
+  // never read)
+  int4 grp = make_int4(0, 0, 0, 0);
+  int4* groups = reinterpret_cast<int4*>(d.nbr);
   if (want && cell_of(sc, xi, cx, cy, cz)) {
     const float thr2 = sc.thr2;
-    const int P = d.P;
+    const long P = d.P;
     const int z0 = cz > 0 ? cz - 1 : 0;
     const int z1 = cz + 1 < sc.gn2 ? cz + 1 : sc.gn2 - 1;
     for (int dx = -1; dx <= 1; dx++) {
@@ -239,7 +244,14 @@ __global__ void k_build_lists(Dev d, int all) {
             const float ss = ex * ex + ey * ey + ez * ez;
             // sqrt_rn(ss) < h  <=>  ss < thr2 (thr2: least float whose rounded sqrt >= h)
             if (ss < thr2) {
-              if (cnt < MAX_NB) d.nbr[(long)cnt * P + i] = j;
+              if (cnt < MAX_NB) {
+                const int w = cnt & 3;
+                grp.x = w == 0 ? j : grp.x;
+                grp.y = w == 1 ? j : grp.y;
+                grp.z = w == 2 ? j : grp.z;
+                grp.w = w == 3 ? j : grp.w;
+                if (w == 3) groups[(long)(cnt >> 2) * P + i] = grp;
+              }
               cnt++;
             }
           }
@@ -250,6 +262,7 @@ __global__ void k_build_lists(Dev d, int all) {
       atomicOr(&d.state[s].status, ST_NEIGHBOR_OVERFLOW);
       cnt = MAX_NB;
     }
+    if (cnt & 3) groups[(long)(cnt >> 2) * P + i] = grp;
   }
   int cf = 0;
   if (i < sc.p_begin + sc.n) {
@@ -282,7 +295,7 @@ __global__ void k_bvol(Dev d, int dynamic) {
   float plain = sc.W0;
   const bool comp = (sc.flags & SF_COMPENSATED) != 0;
   for (int k = 0; k < cnt; k++) {
-    const int j = d.nbr[(long)k * P + i];
+    const int j = d.nbr[nb_word(P, i, k)];
     const float4 xj = d.pos[j];
     if (xj.w < 0.0f) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;

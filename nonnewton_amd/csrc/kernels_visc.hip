@@ -131,14 +131,14 @@ __global__ void k_cg_init(Dev d) {
         m4 += sij * ry * rz;
         m5 += sij * rz * rz;
       }
-      d.spair[k * P + i] = sij;
+      d.spair[nb_word(P, i, k)] = sij;
     };
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     // two list positions of this lane per batch, their loads issued together
     for (int k = q; k < cnt; k += 8) {
       const bool two = k + 4 < cnt;
-      const int j0 = d.nbr[k * P + i];
-      const int j1 = two ? d.nbr[(k + 4) * P + i] : i;
+      const int j0 = d.nbr[nb_word(P, i, k)];
+      const int j1 = two ? d.nbr[nb_word(P, i, k + 4)] : i;
       const float4 x0 = d.pos[j0], x1 = d.pos[j1];
       const float4 v0 = d.vel[j0], v1 = d.vel[j1];
       const float mu0 = d.mu[j0], mu1 = d.mu[j1];
@@ -258,8 +258,8 @@ __global__ void k_cg_matvec(Dev d, int start) {
 #pragma unroll
       for (int e = 0; e < D; e++) {
         const bool in = k + 4 * e < cnt;
-        j[e] = in ? d.nbr[(k + 4 * e) * P + i] : i;
-        sp[e] = in ? d.spair[(k + 4 * e) * P + i] : 0.0f;
+        j[e] = in ? d.nbr[nb_word(P, i, k + 4 * e)] : i;
+        sp[e] = in ? d.spair[nb_word(P, i, k + 4 * e)] : 0.0f;
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
@@ -322,8 +322,8 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
 #pragma unroll
         for (int e = 0; e < D; e++) {
           const bool in = kk + 4 * e < cnt;
-          j[e] = in ? d.nbr[(kk + 4 * e) * P + i] : i;
-          sp[e] = in ? d.spair[(kk + 4 * e) * P + i] : 0.0f;
+          j[e] = in ? d.nbr[nb_word(P, i, kk + 4 * e)] : i;
+          sp[e] = in ? d.spair[nb_word(P, i, kk + 4 * e)] : 0.0f;
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {

@@ -49,6 +49,9 @@ extern "C" {
 #define NN_KIND_FLUID 0
 #define NN_KIND_STATIC 1
 #define NN_KIND_ELASTIC 2
+/* NN_KIND_RIGID + b: a particle of rigid body b (solidType RIGID, RigidBodySolver :1099-1158);
+ * bodies are numbered 0 .. B-1 without gaps, B <= 4096 */
+#define NN_KIND_RIGID 3
 
 /* Scene parameters: Parameter (SPH/main.py:598-621), DFSPHSolver.__init__
  * (:1520-1544), NonNewton.initParameters (:358-371), SurfaceTension (:1382),
@@ -90,8 +93,9 @@ typedef struct nn_params {
     double youngs_modulus;
     double poisson_ratio;
     double particle_mass; /* <= 0: volume0 * 1000 (:743) */
-    int32_t warm_start;   /* reserved, must be 0 */
-    int32_t reserved;
+    int32_t warm_start;   /* bit 0: the Weiler CG starts from v + (x - v) of the previous solve
+                           * (Viscosity_Weiler2018's m_vDiff, :1486-1487); other bits must be 0 */
+    int32_t coupling_interval; /* rigid-body substeps per step (Parameter.coupling_interval, :616) */
 } nn_params;
 
 /* Per-step statistics (what the reference prints each step, :916,1548,1704,1785). */
@@ -104,9 +108,9 @@ typedef struct nn_step_stats {
     double avg_density_err_p;
     double cg_rel_residual;
     double max_density;
-    int64_t num_pairs;       /* sum of neighbor counts of fluid and elastic particles (a step
-                              * builds no lists for static boundary particles; nn_neighbors
-                              * and the NN_FIELD_NUM_NEIGHBORS download complete them) */
+    int64_t num_pairs;       /* sum of neighbor counts of fluid, elastic and rigid particles (the
+                              * lists a step reads; static boundary lists are built on demand
+                              * by nn_neighbors and the NN_FIELD_NUM_NEIGHBORS download) */
     int64_t num_pairs_fluid; /* sum of neighbor counts of fluid particles */
 } nn_step_stats;
 

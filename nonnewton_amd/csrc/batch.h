@@ -18,7 +18,14 @@ int hip_fail(hipError_t e, const char* what);
 
 struct SceneHost {
   nn_params params{};
-  int n = 0, n_fluid = 0, n_elastic = 0;
+  int n = 0, n_fluid = 0, n_elastic = 0, n_rigid = 0;
+  // rigid bodies: original ids of each body in increasing order (body b's ids
+  // are body_ids[body_off[b] .. body_off[b + 1])), rest centres of mass
+  int nbody = 0;
+  std::vector<int> body_ids, body_off;
+  std::vector<float> rest_cm;  // 3 per body
+  int r_begin = 0, nr = 0;     // rigid-set range in its batch (each body padded to 64)
+  int body_begin = 0;          // first entry of the batch's body table
   int gn[3] = {0, 0, 0};
   long ncells = 0;  // grid cells (one more is reserved for out-of-grid particles)
   // layout inside its current batch
@@ -35,6 +42,9 @@ struct SceneHost {
 
 // derive the FP32 constants of a scene from its parameters (no layout fields)
 int derive_scene(SceneHost& s, std::string& err);
+// rest centres of mass of the rigid bodies from the initial positions x
+// (after derive_scene and the kinds' body tables)
+void rigid_rest_centres(SceneHost& s, const float* x);
 // least float T with sqrt_rn(T) >= h
 float neighbor_threshold(float h);
 // device bytes of one scene of n particles, ns stress-set members, c cells
@@ -131,7 +141,10 @@ class Batch {
   SceneState* h_state_ = nullptr;  // pinned
   int* h_active_ = nullptr;        // pinned
   bool any_fluid_ = false, any_elastic_ = false, any_weiler_ = false, any_strain_ = false, any_div_ = false;
-  bool any_warm_cg_ = false;
+  bool any_warm_cg_ = false, any_rigid_ = false;
+  int max_body_ = 0, max_coupling_ = 0;  // over the scenes with rigid bodies
+  int NB_cap_ = 0;                       // body-table entries allocated
+  int rigid_step();
   int max_iter_[3] = {1, 1, 1};
 
   template <class T>

@@ -171,6 +171,9 @@ int derive_scene(SceneHost& s, std::string& err) {
   c.max_iter_v = p.max_iterations_v;
   c.max_iter_p = p.max_iterations;
   c.max_iter_cg = p.weiler_max_iter;
+  c.vol0 = (float)vol0;
+  c.nbody = s.nbody;
+  c.coupling = std::max(0, p.coupling_interval);
   int f = 0;
   if (p.use_surface_tension) f |= SF_SURFACE_TENSION;
   if (p.use_viscosity) f |= SF_VISCOSITY;
@@ -185,12 +188,31 @@ int derive_scene(SceneHost& s, std::string& err) {
   if (p.compensated) f |= SF_COMPENSATED;
   if (s.n_fluid > 0) f |= SF_HAS_FLUID;
   if (s.n_elastic > 0) f |= SF_HAS_ELASTIC;
+  if (s.nbody > 0) f |= SF_HAS_RIGID;
   c.flags = f;
   c.n = s.n;
   c.n_fluid = s.n_fluid;
   c.n_elastic = s.n_elastic;
   c.nf_d = (double)s.n_fluid;
   return NN_OK;
+}
+
+// Rest centres of mass of the rigid bodies (RigidBodySolver.__init__,
+// :1100-1106) from the initial positions: FP64 sums of the FP32 terms m x in
+// index order, rounded once.  Needs derive_scene (the mass) and the bodies of
+// the kinds; runs before the scene is laid out.
+void rigid_rest_centres(SceneHost& s, const float* x) {
+  const float mass_r = s.dev.vol0 * s.dev.rho0;
+  s.rest_cm.assign(3 * (size_t)s.nbody, 0.0f);
+  for (int b = 0; b < s.nbody; b++) {
+    double sm = 0.0, sx[3] = {0.0, 0.0, 0.0};
+    for (int m = s.body_off[b]; m < s.body_off[b + 1]; m++) {
+      const int id = s.body_ids[m];
+      sm += (double)mass_r;
+      for (int c = 0; c < 3; c++) sx[c] += (double)(mass_r * x[3 * id + c]);
+    }
+    for (int c = 0; c < 3; c++) s.rest_cm[3 * b + c] = (float)(sx[c] / sm);
+  }
 }
 
 // bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wx,wr,wu,ww,wp,ws,wd0,x0,vdiff (14 float4),
@@ -249,7 +271,8 @@ void Batch::swap_sort_buffers() {
 // wave range, and the batch-wide feature flags.
 void Batch::derive_members() {
   long base = 0;
-  any_fluid_ = any_elastic_ = any_weiler_ = any_strain_ = any_div_ = any_warm_cg_ = false;
+  any_fluid_ = any_elastic_ = any_weiler_ = any_strain_ = any_div_ = any_warm_cg_ = any_rigid_ = false;
+  max_body_ = max_coupling_ = 0;
   max_iter_[0] = max_iter_[1] = max_iter_[2] = 1;
   for (auto& sp : scenes) {
     SceneHost& s = *sp;
@@ -260,6 +283,7 @@ void Batch::derive_members() {
     s.dev.cell_begin = s.cell_begin;
     s.dev.s_begin = s.s_begin;
     s.dev.ns = s.ns;
+    s.dev.body_begin = s.body_begin;
     base += s.n;
     const int f = s.dev.flags;
     if (f & SF_HAS_FLUID) {
@@ -270,6 +294,11 @@ void Batch::derive_members() {
       if ((f & SF_NON_NEWTONIAN) || ((f & SF_VISCOSITY) && (f & SF_WEILER))) any_strain_ = true;
     }
     if (f & SF_HAS_ELASTIC) any_elastic_ = true;
+    if (f & SF_HAS_RIGID) {
+      any_rigid_ = true;
+      max_body_ = std::max(max_body_, s.dev.nbody);
+      max_coupling_ = std::max(max_coupling_, s.dev.coupling);
+    }
     max_iter_[0] = std::max(max_iter_[0], s.dev.max_iter_v);
     max_iter_[1] = std::max(max_iter_[1], s.dev.max_iter_p);
     max_iter_[2] = std::max(max_iter_[2], s.dev.max_iter_cg);
@@ -292,6 +321,46 @@ int Batch::upload_tables(const Dev& nd) {
       for (int w = s.dev.wave_begin; w < s.dev.wave_end; w += 16) cgb.push_back(make_int2((int)k, w));
   }
   n_cg_blocks_ = (int)cgb.size();
+  // rigid set and body table, rebuilt from the scenes' host data (ranges of
+  // retired scenes map to no body; their table entries get rank -1)
+  if (nd.NRn > 0 || nd.nbodies > 0) {
+    std::vector<int> rp(std::max(nd.NRn, 1), -1), rwb(std::max(nd.NRn / 64, 1), -1);
+    std::vector<BodyDev> bt(std::max(nd.nbodies, 1));
+    for (auto& b : bt) {
+      b = BodyDev{};
+      b.rank = -1;
+    }
+    for (size_t k = 0; k < scenes.size(); k++) {
+      const SceneHost& s = *scenes[k];
+      if (s.rest_cm.size() < 3 * (size_t)s.nbody || s.body_off.size() < (size_t)s.nbody + 1) {
+        set_error("rigid bodies without rest centres");
+        return NN_E_STATE;
+      }
+      int off = s.r_begin;
+      for (int b = 0; b < s.nbody; b++) {
+        const int lo = s.body_off[b], hi = s.body_off[b + 1];
+        for (int m = lo; m < hi; m++) rp[off + m - lo] = s.body_ids[m];
+        const int len = round_up64(hi - lo);
+        for (int w = off / 64; w < (off + len) / 64; w++) rwb[w] = s.body_begin + b;
+        BodyDev& e = bt[s.body_begin + b];
+        e.scene = (int)k;
+        e.rank = b;
+        e.wave_begin = off / 64;
+        e.wave_end = (off + len) / 64;
+        for (int c = 0; c < 3; c++) e.rest_cm[c] = e.cm[c] = s.rest_cm[3 * b + c];
+        for (int q = 0; q < 9; q++) e.R[q] = (q % 4) == 0 ? 1.0f : 0.0f;
+        off += len;
+      }
+    }
+    if (nd.NRn > 0) {
+      HIPCHK(hipMemcpyAsync((void*)nd.r_pid, rp.data(), sizeof(int) * nd.NRn, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync((void*)nd.r_wave_body, rwb.data(), sizeof(int) * (nd.NRn / 64), hipMemcpyHostToDevice, stream));
+    }
+    if (nd.nbodies > 0)
+      HIPCHK(hipMemcpyAsync((void*)nd.bodies, bt.data(), sizeof(BodyDev) * nd.nbodies, hipMemcpyHostToDevice, stream));
+    // the host vectors must outlive the copies
+    HIPCHK(hipStreamSynchronize(stream));
+  }
   if (!cgb.empty())
     HIPCHK(hipMemcpyAsync((void*)nd.cg_blocks, cgb.data(), sizeof(int2) * cgb.size(), hipMemcpyHostToDevice, stream));
   if (!scenes.empty())
@@ -349,8 +418,8 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
     }
   }
   // offsets, packed in member order (committed once the arrays exist)
-  int P = 0, C = 0, NS = 0;
-  std::vector<int> pb(scenes.size()), cb(scenes.size()), sb(scenes.size());
+  int P = 0, C = 0, NS = 0, NR = 0, NB = 0;
+  std::vector<int> pb(scenes.size()), cb(scenes.size()), sb(scenes.size()), rb(scenes.size()), bb(scenes.size());
   for (size_t k = 0; k < scenes.size(); k++) {
     const SceneHost& s = *scenes[k];
     pb[k] = P;
@@ -359,6 +428,10 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
     C += (int)s.ncells + 1;
     sb[k] = NS;
     NS += round_up64(s.ns);
+    rb[k] = NR;
+    NR += s.nr;
+    bb[k] = NB;
+    NB += s.nbody;
   }
   // capacities: the extents, grown by `slack` so that scenes can be appended
   // later without moving the others
@@ -366,6 +439,8 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   const int Pc = round_up64((int)std::ceil(P * slack));
   const int Cc = (int)std::ceil(C * slack);
   const int NSc = round_up64((int)std::ceil(NS * slack));
+  const int NRc = round_up64((int)std::ceil(NR * slack));
+  const int NBc = (int)std::ceil(NB * slack);
   const int Sc = std::max<int>((int)scenes.size(), scene_capacity);
   // new arrays
   Dev nd{};
@@ -380,6 +455,9 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   nd.NS = NSc;
   nd.SW = NSc / 64;
   nd.NSn = NS;
+  nd.NR = NRc;
+  nd.NRn = NR;
+  nd.nbodies = NB;
   nd.nscenes = (int)scenes.size();
   const int ncell_entries = Cc + 1;
   const int nscan = (ncell_entries + 2047) / 2048;
@@ -442,6 +520,14 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.rest, (size_t)NSc * MAX_REST);
   A(nd.rest_cnt, NSc);
   A(nd.rs, 4 * (size_t)NSc);
+  int* rpid;
+  A(rpid, NRc);
+  nd.r_pid = rpid;
+  int* rwb;
+  A(rwb, NRc / 64);
+  nd.r_wave_body = rwb;
+  A(nd.rpartials, 9 * (size_t)(NRc / 64));
+  A(nd.bodies, NBc);
   if (rc != NN_OK) {
     release();
     allocs_.swap(old_allocs);
@@ -452,10 +538,13 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
     scenes[k]->p_begin = pb[k];
     scenes[k]->cell_begin = cb[k];
     scenes[k]->s_begin = sb[k];
+    scenes[k]->r_begin = rb[k];
+    scenes[k]->body_begin = bb[k];
   }
   derive_members();
   C_total_ = C;
   C_cap_ = Cc;
+  NB_cap_ = NBc;
   scene_cap_ = Sc;
   scan_blocks_ = nscan;
   if (!h_state_) {
@@ -512,18 +601,22 @@ int Batch::append_in_place(const std::vector<std::shared_ptr<SceneHost>>& member
     if (std::find(cur.begin(), cur.end(), members[k].get()) != cur.end()) return LAYOUT_NEEDED;
   if ((int)members.size() > scene_cap_) return LAYOUT_NEEDED;
   // offsets of the new scenes, after the last slot, cell and stress slot used
-  int P = d.Pn, C = C_total_, NS = d.NSn;
+  int P = d.Pn, C = C_total_, NS = d.NSn, NR = d.NRn, NB = d.nbodies;
   std::vector<std::pair<int, int>> place(members.size(), {0, 0});  // (p_begin, s_begin) of new scenes
-  std::vector<int> cells(members.size(), 0);
+  std::vector<int> cells(members.size(), 0), rb(members.size(), 0), bb(members.size(), 0);
   for (size_t k = nk; k < members.size(); k++) {
     const SceneHost& s = *members[k];
     place[k] = {P, NS};
     cells[k] = C;
+    rb[k] = NR;
+    bb[k] = NB;
     P += round_up64(s.n);
     C += (int)s.ncells + 1;
     NS += round_up64(s.ns);
+    NR += s.nr;
+    NB += s.nbody;
   }
-  if (P > d.P || C > C_cap_ || NS > d.NS) return LAYOUT_NEEDED;
+  if (P > d.P || C > C_cap_ || NS > d.NS || NR > d.NR || NB > NB_cap_) return LAYOUT_NEEDED;
   // sources of the new scenes
   struct Src {
     const Batch* b;
@@ -541,11 +634,15 @@ int Batch::append_in_place(const std::vector<std::shared_ptr<SceneHost>>& member
     s.p_begin = place[k].first;
     s.s_begin = place[k].second;
     s.cell_begin = cells[k];
+    s.r_begin = rb[k];
+    s.body_begin = bb[k];
   }
   scenes = members;
   derive_members();
   d.Pn = P;
   d.NSn = NS;
+  d.NRn = NR;
+  d.nbodies = NB;
   d.nscenes = (int)scenes.size();
   C_total_ = C;
   d.C = C;
@@ -580,11 +677,11 @@ int Batch::refresh_scene_table() {
 int Batch::upload(int k, const float* x, const float* v, const int* kind) {
   SceneHost& s = *scenes[k];
   const int n = s.n, pb = s.p_begin;
-  const float V0 = (float)(0.8 * std::pow(2.0 * s.params.particle_radius, 3));  // :607, :741
+  const float V0 = s.dev.vol0;  // :607, :741
   std::vector<float4> hp(n), hv(n);
   std::vector<int> hi(n), hinv(n);
   for (int i = 0; i < n; i++) {
-    const int kd = kind[i];
+    const int kd = kind[i] >= KIND_RIGID ? KIND_RIGID : kind[i];
     hp[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], kd == KIND_FLUID ? V0 : -V0);
     hv[i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     hi[i] = (kd << 28) | i;
@@ -804,6 +901,23 @@ int Batch::init_scenes(const std::vector<int>& which) {
   return NN_OK;
 }
 
+// RigidBodySolver.substep for every scene's bodies (:929-931): body rank r
+// of all scenes together, coupling_interval substeps each (a scene with fewer
+// bodies or substeps sits the extra launches out).
+int Batch::rigid_step() {
+  for (int r = 0; r < max_body_; r++)
+    for (int c = 0; c < max_coupling_; c++) {
+      launch_rigid_ground(dref(), r, c, stream);
+      launch_rigid_sum(dref(), 0, r, c, stream);
+      launch_body_finish(dref(), 0, r, c, stream);
+      launch_rigid_sum(dref(), 1, r, c, stream);
+      launch_body_finish(dref(), 1, r, c, stream);
+      launch_rigid_place(dref(), r, c, stream);
+    }
+  HIPCHK(hipGetLastError());
+  return NN_OK;
+}
+
 // Runs solver iterations until no scene is active: the first chunk is the
 // previous step's count, later chunks two at a time; each chunk ends with one
 // 4-byte read of the active-scene counter.
@@ -893,8 +1007,11 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
       return NN_E_STATE;
     }
   launch_step_begin(dref(), stream);
-  CHK(search_impl(0));                                                 // :917
-  if (any_elastic_) TIMED(K_BVOL, launch_bvol(dref(), blk(K_BVOL), 1, stream));  // :918
+  // static particles get no list inside a step, except in batches with rigid
+  // bodies: their ground collision moves positions after the search, so the
+  // search positions would not survive to build those lists on demand
+  CHK(search_impl(any_rigid_ ? 1 : 0));                                // :917
+  if (any_elastic_ || any_rigid_) TIMED(K_BVOL, launch_bvol(dref(), blk(K_BVOL), 1, stream));  // :918
   TIMED(K_CLEAR_ACC, launch_clear_acc(dref(), blk(K_CLEAR_ACC), stream));             // :920
   if (any_fluid_) {  // DFSPHSolver.substep, :1546-1555
     TIMED(K_DENSITY_ALPHA, launch_density_alpha(dref(), blk(K_DENSITY_ALPHA), dep(K_DENSITY_ALPHA), stream));
@@ -922,6 +1039,7 @@ int Batch::step(std::vector<nn_step_stats>* stats) {
     TIMED(K_ELASTIC_ROT_STRESS, launch_elastic_rot_stress(dref(), blk(K_ELASTIC_ROT_STRESS), stream));
     TIMED(K_ELASTIC_FORCE, launch_elastic_force(dref(), blk(K_ELASTIC_FORCE), stream));
   }
+  if (any_rigid_) CHK(rigid_step());  // :929-931
   TIMED(K_ADVECT, launch_advect(dref(), blk(K_ADVECT), stream));  // :933, :936-937
   std::swap(d.pos, d.pos_alt);  // pos_alt keeps the positions of this step's search
   HIPCHK(hipGetLastError());
@@ -1090,6 +1208,8 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
     case NN_FIELD_KIND:
       HIPCHK(hipStreamSynchronize(stream));
       for (int t = 0; t < n; t++) iout[info[t] & 0x0FFFFFFF] = info[t] >> 28;
+      for (int b = 0; b < s.nbody; b++)  // a rigid particle reads back as KIND_RIGID + its body
+        for (int m = s.body_off[b]; m < s.body_off[b + 1]; m++) iout[s.body_ids[m]] = KIND_RIGID + b;
       return NN_OK;
     case NN_FIELD_NUM_NEIGHBORS: {
       i1.resize(n);
@@ -1154,7 +1274,8 @@ int Batch::upload_field(int k, int field, const void* host, size_t bytes) {
 
 // After a step, the static particles' lists are built on demand, from the
 // positions and cell order of that step's search (kept in pos_alt): the lists
-// the reference holds after a step.
+// the reference holds after a step.  (A step of a batch with rigid bodies
+// builds them in its search.)
 int Batch::complete_lists() {
   if (lists_state_ == 2) return NN_OK;
   if (lists_state_ == 0) return search_impl(1);  // laid out since: search the current positions

@@ -21,18 +21,40 @@ struct nn_ctx {
   bool uploaded = false;
 };
 
+// Kinds and rigid bodies of a scene: kind NN_KIND_RIGID + b is a particle of
+// body b; bodies are numbered 0 .. B-1 without gaps.
 static int count_kinds(SceneHost& s, const int32_t* kind, int64_t n) {
   s.n = (int)n;
-  s.n_fluid = s.n_elastic = 0;
+  s.n_fluid = s.n_elastic = s.n_rigid = 0;
+  s.nbody = 0;
   for (int64_t i = 0; i < n; i++) {
     const int k = kind[i];
     if (k == NN_KIND_FLUID) s.n_fluid++;
     else if (k == NN_KIND_ELASTIC) s.n_elastic++;
-    else if (k != NN_KIND_STATIC) {
-      nn::set_error("kind must be NN_KIND_FLUID, NN_KIND_STATIC or NN_KIND_ELASTIC (rigid bodies are not built)");
-      return k == 3 ? NN_E_UNSUPPORTED : NN_E_ARG;
+    else if (k >= NN_KIND_RIGID && k < NN_KIND_RIGID + nn::MAX_BODIES) {
+      s.n_rigid++;
+      s.nbody = std::max(s.nbody, k - NN_KIND_RIGID + 1);
+    } else if (k != NN_KIND_STATIC) {
+      nn::set_error("kind must be NN_KIND_FLUID, NN_KIND_STATIC, NN_KIND_ELASTIC or NN_KIND_RIGID + body (body < 4096)");
+      return NN_E_ARG;
     }
   }
+  s.body_off.assign(s.nbody + 1, 0);
+  for (int64_t i = 0; i < n; i++)
+    if (kind[i] >= NN_KIND_RIGID) s.body_off[kind[i] - NN_KIND_RIGID + 1]++;
+  s.nr = 0;
+  for (int b = 0; b < s.nbody; b++) {
+    if (s.body_off[b + 1] == 0) {
+      nn::set_error("rigid bodies must be numbered 0 .. B-1 without gaps");
+      return NN_E_ARG;
+    }
+    s.nr += (s.body_off[b + 1] + 63) / 64 * 64;
+    s.body_off[b + 1] += s.body_off[b];
+  }
+  s.body_ids.assign(s.n_rigid, 0);
+  std::vector<int> fill(s.body_off.begin(), s.body_off.end() - (s.nbody > 0 ? 1 : 0));
+  for (int64_t i = 0; i < n; i++)
+    if (kind[i] >= NN_KIND_RIGID) s.body_ids[fill[kind[i] - NN_KIND_RIGID]++] = (int)i;
   return NN_OK;
 }
 
@@ -112,6 +134,7 @@ int32_t nn_upload(nn_ctx* ctx, const float* x, const float* v, const int32_t* ki
     nn::set_error(err);
     return rc;
   }
+  nn::rigid_rest_centres(s, x);
   s.ns = 0;
   s.s_pid.clear();
   s.rest.clear();
@@ -201,6 +224,7 @@ static int apply_param(SceneHost& s, const char* key, double value) {
   s.dev.cell_begin = keep.cell_begin;
   s.dev.s_begin = keep.s_begin;
   s.dev.ns = keep.ns;
+  s.dev.body_begin = keep.body_begin;
   return NN_OK;
 }
 
@@ -386,6 +410,7 @@ int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const
     nn::set_error(err);
     return rc;
   }
+  nn::rigid_rest_centres(*sc, x);
   const long long ns_est = sc->n_elastic > 0 ? sc->n_elastic + sc->n_elastic / 2 + 64 : 0;
   const long long fp = nn::scene_footprint(n, ns_est, sc->ncells);
   int s_id = -1;

@@ -24,6 +24,8 @@ __device__ __forceinline__ bool slot_scene(const Dev& d, int& i, int& s) {
 
 __device__ __forceinline__ int kind_of(int info) { return info >> 28; }
 __device__ __forceinline__ int id_of(int info) { return info & 0x0FFFFFFF; }
+// dynamic solids (SOLID and is_dynamic, SPH/main.py:281-282): elastic and rigid particles
+__device__ __forceinline__ bool dyn_solid(int kind) { return kind == KIND_ELASTIC || kind == KIND_RIGID; }
 
 // ---------------------------------------------------------------------------
 // SPH kernel, SPH/main.py:859-880 (cubic_kernel) and :883-905 (derivative)

@@ -32,7 +32,8 @@ constexpr int MAX_PIC = 50;   // max_num_particles_in_grid, SPH/main.py:778
 constexpr int MAX_REST = 50;  // Elasticity.max_num_neighbors, SPH/main.py:1192
 constexpr int WAVE = 64;
 
-enum { KIND_FLUID = 0, KIND_STATIC = 1, KIND_ELASTIC = 2 };
+enum { KIND_FLUID = 0, KIND_STATIC = 1, KIND_ELASTIC = 2, KIND_RIGID = 3 };
+constexpr int MAX_BODIES = 4096;  // rigid bodies per scene
 enum {
   ST_NEIGHBOR_OVERFLOW = 1,
   ST_CELL_OVERFLOW = 2,
@@ -50,7 +51,8 @@ enum {
   SF_COMPENSATED = 32,
   SF_HAS_FLUID = 64,
   SF_HAS_ELASTIC = 128,
-  SF_WARM_CG = 256  // Weiler CG starts from v + (x - v) of the previous solve
+  SF_WARM_CG = 256,  // Weiler CG starts from v + (x - v) of the previous solve
+  SF_HAS_RIGID = 512
 };
 
 // Per-scene constants (device table).  Floats are the reference's FP32
@@ -62,6 +64,7 @@ struct SceneDev {
   int wave_begin, wave_end;
   int s_begin, ns;  // stress-set range
   int flags, nn_method;
+  int nbody, body_begin, coupling;  // rigid bodies: count, first entry of d.bodies, substeps per step
   int max_iter_v, max_iter_p, max_iter_cg;
   float h, thr2, kW, kG, W0;
   float dt, inv_dt, inv_dt2, rho0, mass;
@@ -71,8 +74,19 @@ struct SceneDev {
   float C_diag, C_off, C_shear;
   float nn_n, nn_K, nn_e0, nn_einf, nn_gc, nn_muc, nn_tauy, nn_max;
   float ten_nu, ten_bnu, sig_m;  // 10*nu, 10*bnu, (sigma/m)*m
-  float pad0;
+  float vol0;  // volume0 = 0.8 (2r)^3 (:607)
   double eta_v, eta_p, cg_tol2, nf_d, rho0_d;
+};
+
+// A rigid body (RigidBodySolver, SPH/main.py:1099-1158).  Its particles are
+// the rigid-set slots of waves [wave_begin, wave_end) (original ids in r_pid,
+// -1 on padding); `rank` is its index among its scene's bodies.
+struct BodyDev {
+  int scene, rank, wave_begin, wave_end;
+  float rest_cm[3];  // centre of mass at construction (:1105)
+  float cm[3];       // of the current projection
+  float R[9];        // rotation of the current projection, row-major
+  float pad;
 };
 
 // Per-scene mutable solver state.
@@ -98,6 +112,9 @@ struct Dev {
   int NS;       // stress-set slots allocated (each scene's range padded to 64; the rest-list stride)
   int SW;       // stress-set waves allocated = NS / 64
   int NSn;      // stress-set slots in use
+  int NR;       // rigid-set slots allocated (each body's range padded to 64)
+  int NRn;      // rigid-set slots in use
+  int nbodies;  // entries of bodies in use
   // A wave (or stress-set wave) that belongs to no scene -- the range of a
   // retired scene, or slack past the last one -- has scene index -1 and is
   // skipped by every sweep.
@@ -151,6 +168,11 @@ struct Dev {
   int* rest;    // rest[k * NS + q]: local id of the k-th rest neighbor
   int* rest_cnt;
   float4* rs;   // 4 per member: rows of R with stress 0..2 in w, then (s3, s4, s5, 0)
+  // rigid bodies (rigid set: each body's particles by original id, padded to 64)
+  const int* r_pid;        // rigid-set slot -> local id (-1 on padding)
+  const int* r_wave_body;  // rigid-set wave -> entry of bodies (-1: none)
+  BodyDev* bodies;
+  double* rpartials;       // 9 arrays of NR / 64: rigid-set wave partials
 };
 
 // ---------------------------------------------------------------------------
@@ -192,6 +214,12 @@ void launch_clear_acc(const DevRef& d, int block, hipStream_t st);
 void launch_elastic_init(const DevRef& d, int block, hipStream_t st);
 void launch_elastic_rot_stress(const DevRef& d, int block, hipStream_t st);
 void launch_elastic_force(const DevRef& d, int block, hipStream_t st);
+// rigid body r of every scene, substep c (RigidBodySolver.substep, :1112-1115):
+// ground collision, centre of mass, A and its rotation, projection
+void launch_rigid_ground(const DevRef& d, int r, int c, hipStream_t st);
+void launch_rigid_sum(const DevRef& d, int pass, int r, int c, hipStream_t st);
+void launch_body_finish(const DevRef& d, int pass, int r, int c, hipStream_t st);
+void launch_rigid_place(const DevRef& d, int r, int c, hipStream_t st);
 void launch_advect(const DevRef& d, int block, hipStream_t st);
 void launch_step_begin(const DevRef& d, hipStream_t st);
 void launch_scene_finish(const DevRef& d, int which, hipStream_t st);  // 0 div, 1 press, 2 cg start, 3 cg iteration, 4 cg guess scale

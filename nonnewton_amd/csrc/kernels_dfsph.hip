@@ -302,7 +302,7 @@ __global__ void k_nonpressure(Dev d) {
     d.vel_alt[i] = make_float4(vi.x + sc.dt * ax, vi.y + sc.dt * ay, vi.z + sc.dt * az, vi.w);
     return;
   }
-  if (kind == KIND_ELASTIC) {
+  if (dyn_solid(kind)) {
     float ax = sc.gx, ay = sc.gy, az = sc.gz;
     if (bnu_on) {  // reaction of the boundary viscosity (:1458-1460), gathered
       const float4 xi = d.pos[i];
@@ -490,7 +490,7 @@ __global__ void k_press_update(Dev d) {
           }
         });
     d.vel[i] = v;
-  } else if (kind == KIND_ELASTIC) {
+  } else if (dyn_solid(kind)) {
     const float Vi = -xi.w;
     float4 a = d.acc[i];
     for (int k = 0; k < cnt; k++) {
@@ -527,7 +527,7 @@ __global__ void k_advect(Dev d) {
     return;
   }
   float4 v = d.vel[i];
-  if (kind == KIND_ELASTIC) {
+  if (dyn_solid(kind)) {
     const float4 a = d.acc[i];
     v.x += sc.dt * a.x;
     v.y += sc.dt * a.y;

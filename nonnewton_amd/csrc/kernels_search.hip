@@ -264,12 +264,15 @@ __global__ void k_build_lists(Dev d, int all) {
     }
     if (cnt & 3) groups[(long)(cnt >> 2) * P + i] = grp;
   }
-  int cf = 0;
+  // the step's pair counts cover the lists a step uses: fluid and dynamic
+  // solid particles (static boundary lists serve only the boundary volumes)
+  int cf = 0, tot = 0;
   if (i < sc.p_begin + sc.n) {
     d.nnbr[i] = cnt;
-    if (kind_of(d.info[i]) == KIND_FLUID) cf = cnt;
+    const int kind = kind_of(d.info[i]);
+    if (kind == KIND_FLUID) cf = cnt;
+    if (kind != KIND_STATIC) tot = cnt;
   }
-  int tot = cnt;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     tot += __shfl_xor(tot, off, 64);
@@ -287,7 +290,7 @@ __global__ void k_bvol(Dev d, int dynamic) {
   const SceneDev& sc = d.scenes[s];
   if (i >= sc.p_begin + sc.n) return;
   const int kind = kind_of(d.info[i]);
-  if (dynamic ? kind != KIND_ELASTIC : kind != KIND_STATIC) return;
+  if (dynamic ? !dyn_solid(kind) : kind != KIND_STATIC) return;
   const float4 xi = d.pos[i];
   const int cnt = d.nnbr[i];
   const int P = d.P;

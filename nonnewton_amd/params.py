@@ -25,6 +25,7 @@ ELASTIC = 2
 KIND_FLUID = 0
 KIND_STATIC = 1
 KIND_ELASTIC = 2
+KIND_RIGID = 3  # KIND_RIGID + b: a particle of rigid body b
 
 NON_NEWTON_METHODS = {
     "NEWTONIAN": 0,
@@ -90,7 +91,9 @@ class SceneParams:
     # initial guess of the published solver; the reference declares its
     # m_vDiff fields, SPH/main.py:1486-1487, with an empty solver around them)
     warm_start: int = 1
-    reserved: int = 0
+    # rigid-body substeps (ground collision + shape matching) per step,
+    # Parameter.coupling_interval, SPH/main.py:616 ("couplingInterval")
+    coupling_interval: int = 1
 
     # ---- derived values, as the reference computes them in Python double ----
     @property
@@ -164,7 +167,7 @@ C_FIELDS: List[Tuple[str, str]] = [
     ("poisson_ratio", "d"),
     ("particle_mass", "d"),
     ("warm_start", "i"),
-    ("reserved", "i"),
+    ("coupling_interval", "i"),
 ]
 
 
@@ -219,11 +222,11 @@ _CONFIG_KEYS = {
     "useNonNewtonianModel": "use_non_newtonian",
     "useViscosityModel": "use_viscosity",
     "enableDivergenceSolver": "enable_divergence_solver",
+    "couplingInterval": "coupling_interval",
 }
 # keys the reference reads but that do not change the step this build runs
 _CONFIG_KEYS_HOST = {
     "simulationMethod",
-    "couplingInterval",
     "solidTimeStepSize",
     "useDragForceModel",
     "useElasticityModel",

@@ -2,7 +2,7 @@
 
 All generators use NumPy's PCG64 and return a :class:`Scene`: parameters plus
 FP32 positions, velocities and particle kinds in the reference's particle order
-(fluid, then static solids, then elastic solids; SPH/main.py:627-666).  Lattices
+(fluid, then static solids, rigid bodies, elastic solids; SPH/main.py:627-666).  Lattices
 are filled with z fastest, then y, then x, the order of the reference's
 ``cube.ply``.  "Jitter +-1%" is a per-axis uniform offset of +-1% of the spacing.
 
@@ -10,6 +10,7 @@ Scenes (sizes in particles; spacing d = 2r = 0.02 m, h = 4r = 0.04 m):
   dam_break          24^3 = 13,824 fluid + 8,706 wall, dt 1e-3, explicit viscosity 0.01
   nonnewton_dam      64x32x64 = 131,072 fluid + walls, Cross model, Weiler CG, dt 5e-4
   elastic_drop       48x32x64 = 98,304 elastic + 40,000 plate, E 1e5, nu 0.33, dt 2e-4
+  rigid_drop         24x12x24 = 6,912 fluid + an 8^3 rigid cube + walls, dt 5e-4
   mixed_batch        64 scenes, dynamic count log-uniform 2k-60k, seed 3
   serving_requests   Poisson request stream, 1k-500k particles, 50-500 steps, seed 4
 """
@@ -21,7 +22,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from .params import KIND_ELASTIC, KIND_FLUID, KIND_STATIC, VISCOSITY_WEILER, SceneParams
+from .params import KIND_ELASTIC, KIND_FLUID, KIND_RIGID, KIND_STATIC, VISCOSITY_WEILER, SceneParams
 
 SPACING = 0.02  # 2r with r = 0.01
 
@@ -53,8 +54,12 @@ class Scene:
         return int(np.count_nonzero(self.kind == KIND_STATIC))
 
     @property
+    def n_rigid(self) -> int:
+        return int(np.count_nonzero(self.kind >= KIND_RIGID))
+
+    @property
     def n_dynamic(self) -> int:
-        return self.n_fluid + self.n_elastic
+        return self.n_fluid + self.n_elastic + self.n_rigid
 
 
 def lattice(origin, counts, spacing=SPACING) -> np.ndarray:
@@ -81,13 +86,18 @@ def shell(domain_cells: Tuple[int, int, int], spacing=SPACING) -> np.ndarray:
     return pts[on]
 
 
-def assemble(name, params, fluid, static, elastic, steps, meta=None) -> Scene:
-    parts = [p for p in (fluid, static, elastic) if p is not None and len(p)]
+def assemble(name, params, fluid, static, elastic, steps, meta=None, rigid=None) -> Scene:
+    """Particles in the reference's phase order (SPH/main.py:627-666): fluid,
+    static, rigid bodies (`rigid`: a list of arrays, body b gets kind
+    KIND_RIGID + b), elastic."""
+    rigid = [r for r in (rigid or []) if len(r)]
+    parts = [p for p in [fluid, static] + rigid + [elastic] if p is not None and len(p)]
     x = np.concatenate(parts, axis=0).astype(np.float32)
     kind = np.concatenate(
         [np.full(len(fluid) if fluid is not None else 0, KIND_FLUID, np.int32),
-         np.full(len(static) if static is not None else 0, KIND_STATIC, np.int32),
-         np.full(len(elastic) if elastic is not None else 0, KIND_ELASTIC, np.int32)]
+         np.full(len(static) if static is not None else 0, KIND_STATIC, np.int32)]
+        + [np.full(len(r), KIND_RIGID + b, np.int32) for b, r in enumerate(rigid)]
+        + [np.full(len(elastic) if elastic is not None else 0, KIND_ELASTIC, np.int32)]
     )
     v = np.zeros_like(x)
     return Scene(name=name, params=params, x=x, v=v, kind=kind, steps=steps, meta=meta or {})
@@ -157,6 +167,27 @@ def elastic_drop(seed: int = 2, block_counts=(48, 32, 64), plate_counts=(200, 20
     # bottom layer 0.05 m above the plate: farther than h, so no rest neighbor is a plate particle
     block = jitter(rng, lattice((ox, SPACING + 0.05, oz), block_counts))
     return assemble("elastic_drop", p, None, plate, block, steps)
+
+
+def rigid_drop(seed: int = 5, fluid_counts=(24, 12, 24), cube=8, steps: int = 200, dt: float = 5e-4) -> Scene:
+    """Two-way coupling (the reference's test_oneway_coupling.json layout): a
+    jittered fluid pool on the floor of a walled box and a rigid cube of
+    cube^3 particles released above it; explicit viscosity, coupling
+    interval 1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    fx, fy, fz = fluid_counts
+    cells = (fx + 8, fy + cube + 14, fz + 8)
+    p = SceneParams(
+        domain_end=tuple(c * SPACING for c in cells),
+        gravity=(0.0, -9.81, 0.0),
+        dt=dt,
+        coupling_interval=1,
+    )
+    fl = jitter(rng, lattice((4 * SPACING, 3 * SPACING, 4 * SPACING), fluid_counts))
+    c0 = ((fx + 8 - cube) / 2.0 * SPACING, (fy + 6) * SPACING, (fz + 8 - cube) / 2.0 * SPACING)
+    body = lattice(c0, (cube, cube, cube))
+    walls = shell(cells)
+    return assemble("rigid_drop", p, fl, walls, None, steps, meta={"cube": cube}, rigid=[body])
 
 
 def _fluid_block_dims(n: int) -> Tuple[int, int, int]:
@@ -273,6 +304,7 @@ def by_name(name: str, **kw):
         "dam_break": dam_break,
         "nonnewton_dam": nonnewton_dam,
         "elastic_drop": elastic_drop,
+        "rigid_drop": rigid_drop,
         "mixed_batch": mixed_batch,
         "serving_requests": serving_requests,
         "tiny_box": tiny_box,

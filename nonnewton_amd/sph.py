@@ -9,8 +9,9 @@ libnnsph.so through ctypes; there is no host or CPU fallback.
 
 Differences from the reference, by design:
   * ``SimConfig`` takes the scene path (no tkinter dialog, SPH/main.py:47-57).
-  * Rigid bodies (solidType RIGID, :498-591, :1099-1158) are not built:
-    ``initialize`` raises NotImplementedError for them.
+  * Rigid bodies (solidType RIGID) run the reference's RigidBodySolver
+    (:1099-1158) inside ``solver.step()``; ``meta.rbs`` lists them.  The unused
+    RigidBody class (:498-591) is not mirrored.
   * Per-step ``print`` output (:916, :1548, :1704, :1785) is kept in
     ``solver.last_stats`` instead.
 """
@@ -26,8 +27,8 @@ import numpy as np
 
 from . import _lib
 from . import ply as _ply
-from .params import (ELASTIC, FLUID, KIND_ELASTIC, KIND_FLUID, KIND_STATIC, RIGID, SOLID, STATIC, SceneParams,
-                     params_from_config)
+from .params import (ELASTIC, FLUID, KIND_ELASTIC, KIND_FLUID, KIND_RIGID, KIND_STATIC, RIGID, SOLID, STATIC,
+                     SceneParams, params_from_config)
 
 RED = (1, 0, 0)
 GREEN = (0, 1, 0)
@@ -266,8 +267,6 @@ def load_particles():
         cnt = 0
         for c in solids:
             if solid_type_selector(c) == st:
-                if st == RIGID:
-                    raise NotImplementedError("rigid bodies (solidType RIGID) are not built")
                 cnt = parse(c, cnt, start, SOLID, st, color, dyn)
     return counter["n"], fluid
 
@@ -313,13 +312,22 @@ def initialize_particles(pd: ParticleData):
     dyn = np.ones(n, np.int32)
     phase = np.zeros(n, np.int32)
     color = np.ones((n, 3), np.float32)
+    body = 0  # rigid bodies are numbered in phase order, the order of meta.rbs (:1882-1887)
     for ph in meta.phase_info.values():
         s, e = ph.startnum, ph.startnum + ph.parnum
         material[s:e] = ph.material
         dyn[s:e] = int(ph.is_dynamic)
         phase[s:e] = ph.uid
         color[s:e] = ph.color
-        pd.kind[s:e] = KIND_FLUID if ph.material == FLUID else (KIND_ELASTIC if ph.solid_type == ELASTIC else KIND_STATIC)
+        if ph.material == FLUID:
+            pd.kind[s:e] = KIND_FLUID
+        elif ph.solid_type == ELASTIC:
+            pd.kind[s:e] = KIND_ELASTIC
+        elif ph.solid_type == RIGID:
+            pd.kind[s:e] = KIND_RIGID + body
+            body += 1
+        else:
+            pd.kind[s:e] = KIND_STATIC
     pd.material.from_numpy(material)
     pd.is_dynamic.from_numpy(dyn)
     pd.phase_id.from_numpy(phase)
@@ -392,6 +400,16 @@ class Elasticity:
         self.m_rotations = DeviceField(_lib.FIELD_ROTATION, numParticles, 9)
         self.m_stress = DeviceField(_lib.FIELD_STRESS, numParticles, 6)
         self.m_restVolumes = DeviceField(_lib.FIELD_REST_VOLUME, numParticles)
+
+
+class RigidBodySolver:
+    """One rigid body (RigidBodySolver, SPH/main.py:1099-1158).  Its substep --
+    ground collision and shape matching, coupling_interval times -- runs on the
+    GPU inside ``solver.step()``, after elasticity and before advection."""
+
+    def __init__(self, phase_id, body):
+        self.phase_id = phase_id
+        self.body = body  # the body's number in the particle kinds (KIND_RIGID + body)
 
 
 class SPHBase:
@@ -483,7 +501,9 @@ def initialize(scene_path: str = "", scene=None, params: Optional[SceneParams] =
     meta.ctx = ctx
     meta.ns = NeighborhoodSearchSparse(meta.pd.x, meta.particle_max_num, meta.parm.support_radius,
                                       meta.parm.domain_size, use_sparse_grid=True)
-    meta.rbs = []
+    n_body = int(meta.pd.kind.max()) - KIND_RIGID + 1 if meta.pd.kind.size and meta.pd.kind.max() >= KIND_RIGID else 0
+    rigid_phases = [ph.uid for ph in getattr(meta, "phase_info", {}).values() if ph.solid_type == RIGID and ph.material == SOLID]
+    meta.rbs = [RigidBodySolver(rigid_phases[b] if b < len(rigid_phases) else None, b) for b in range(n_body)]
     n_el = int(np.count_nonzero(meta.pd.kind == KIND_ELASTIC))
     meta.elastic_bodies = [Elasticity(meta.particle_max_num)] if n_el else []
     meta.num_elastic_bodies = len(meta.elastic_bodies)

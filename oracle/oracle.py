@@ -88,6 +88,9 @@ def _lib(precision: str):
     lib.or_weiler_probe.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
                                     ctypes.c_int32, P(OrStats)]
     lib.or_weiler_probe.restype = ctypes.c_int32
+    lib.or_rigid_frame.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double),
+                                   P(ctypes.c_double)]
+    lib.or_rigid_frame.restype = ctypes.c_int32
     lib.or_num_threads.restype = ctypes.c_int32
     lib.or_set_threads.argtypes = [ctypes.c_int32]
     _LIBS[precision] = lib
@@ -160,6 +163,17 @@ class OracleSim:
         cnt = np.zeros(self.n, dtype=np.int32)
         self.lib.or_neighbors(self.h, _ptr(nbr, ctypes.c_int32), _ptr(cnt, ctypes.c_int32))
         return nbr, cnt
+
+    def rigid_frame(self, body: int):
+        """(rest centre of mass, centre of mass, rotation) of rigid body `body`
+        after its last shape-matching projection."""
+        c0 = np.zeros(3)
+        cm = np.zeros(3)
+        R = np.zeros(9)
+        if self.lib.or_rigid_frame(self.h, int(body), _ptr(c0, ctypes.c_double), _ptr(cm, ctypes.c_double),
+                                   _ptr(R, ctypes.c_double)) != 0:
+            raise ValueError(f"no rigid body {body}")
+        return c0, cm, R.reshape(3, 3)
 
     def weiler_probe(self):
         nf = int(np.count_nonzero(self.kind == 0))

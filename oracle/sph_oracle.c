@@ -91,6 +91,11 @@ struct or_sim {
     /* Weiler CG work vectors (fluid particles, 3 per particle) */
     real *wx, *wr, *wz, *wp, *wq, *ws, *wdinv, *wmu, *wpair;
     real *wvd; /* x - v of the previous solve (initial guess with warm_start bit 0) */
+    /* rigid bodies: body index of each particle (-1 if none); per body the
+     * rest centre of mass and the last projection's centre and rotation */
+    int nbodies;
+    int32_t *body;
+    real *rb_rest_cm, *rb_cm, *rb_R;
     int initialized;
 };
 
@@ -99,7 +104,9 @@ struct or_sim {
 /* ------------------------------------------------------------------------ */
 static inline int is_fluid(const or_sim *s, int i) { return s->kind[i] == OR_KIND_FLUID; }
 static inline int is_static_solid(const or_sim *s, int i) { return s->kind[i] == OR_KIND_STATIC; }
-static inline int is_dyn_solid(const or_sim *s, int i) { return s->kind[i] == OR_KIND_ELASTIC; }
+/* dynamic solids (SOLID and is_dynamic, :281-282): elastic and rigid particles */
+static inline int is_dyn_solid(const or_sim *s, int i) { return s->kind[i] == OR_KIND_ELASTIC || s->kind[i] == OR_KIND_RIGID; }
+static inline int is_elastic(const or_sim *s, int i) { return s->kind[i] == OR_KIND_ELASTIC; }
 
 typedef struct { real s, c; } ksum;
 /* Kahan (compensated) summation step. */
@@ -973,7 +980,7 @@ static void elastic_init(or_sim *s)
     /* stress set S: elastic particles and every rest neighbor of one */
     char *mark = (char *)calloc(s->n, 1);
     for (int i = 0; i < s->n; i++) {
-        if (!is_dyn_solid(s, i)) continue;
+        if (!is_elastic(s, i)) continue;
         mark[i] = 1;
         FOR_NB(s, i, k, j) mark[j] = 1;
     }
@@ -1071,7 +1078,7 @@ static void elastic_forces(or_sim *s)
 #pragma omp parallel for schedule(static)
     for (int q = 0; q < s->ns; q++) {
         int i = s->s_pid[q];
-        if (!is_dyn_solid(s, i)) continue;
+        if (!is_elastic(s, i)) continue;
         const real *xi0 = s->x0 + 3 * i;
         const int32_t *rl = s->rest + (long)q * MAX_REST;
         const real *Ri = s->R + 9 * q, *si = s->stress + 6 * q;
@@ -1156,6 +1163,98 @@ static int32_t clamp_boundary(or_sim *s)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Rigid bodies: RigidBodySolver, :1099-1158                                  */
+/* ------------------------------------------------------------------------ */
+/* compute_com, :1120-1130: mass-weighted centre of body b, mass = volume0 *
+ * density (a solid's density keeps its initial density0, :744; only fluid
+ * densities are recomputed).  The reference adds the FP32 terms m x and m with
+ * atomics in an unspecified order; here they are added in FP64 in index order
+ * and the quotient is rounded once. */
+static void rigid_com(const or_sim *s, int b, real cm[3])
+{
+    double sm = 0.0, sx[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < s->n; i++) {
+        if (s->body[i] != b) continue;
+        real m = s->vol0 * s->rho[i];
+        sm += (double)m;
+        for (int c = 0; c < 3; c++) sx[c] += (double)(m * s->x[3 * i + c]);
+    }
+    for (int c = 0; c < 3; c++) cm[c] = (real)(sx[c] / sm);
+}
+
+/* ground_collision, :1117-1122: every solid particle (static, elastic or
+ * rigid, of any body) below y = padding is lifted to y = padding. */
+static void rigid_ground(or_sim *s)
+{
+    for (int i = 0; i < s->n; i++)
+        if (!is_fluid(s, i) && s->x[3 * i + 1] < s->h) s->x[3 * i + 1] = s->h;
+}
+
+/* solve_constraints, :1138-1158: A = sum m (x - cm)(x0 - cm0)^T (FP64 sums in
+ * index order, rounded once), R = polar(A) (identity if every entry is below
+ * 1e-6), then x += (cm + R (x0 - cm0) - x) * 1.0. */
+static void rigid_project(or_sim *s, int b)
+{
+    real cm[3];
+    rigid_com(s, b, cm);
+    const real *c0 = s->rb_rest_cm + 3 * b;
+    double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < s->n; i++) {
+        if (s->body[i] != b) continue;
+        real m = s->vol0 * s->rho[i];
+        real q[3], p[3];
+        for (int c = 0; c < 3; c++) {
+            q[c] = s->x0[3 * i + c] - c0[c];
+            p[c] = s->x[3 * i + c] - cm[c];
+        }
+        for (int a = 0; a < 3; a++)
+            for (int e = 0; e < 3; e++) A[3 * a + e] += (double)(m * (p[a] * q[e]));
+    }
+    real Af[9], R[9];
+    for (int k = 0; k < 9; k++) Af[k] = (real)A[k];
+    polar3(Af, R);
+    int tiny = 1;
+    for (int k = 0; k < 9; k++)
+        if (!(RFABS(R[k]) < (real)1e-6)) tiny = 0;
+    if (tiny)
+        for (int k = 0; k < 9; k++) R[k] = (k % 4) == 0 ? (real)1.0 : (real)0.0;
+    for (int i = 0; i < s->n; i++) {
+        if (s->body[i] != b) continue;
+        real q[3];
+        for (int c = 0; c < 3; c++) q[c] = s->x0[3 * i + c] - c0[c];
+        for (int a = 0; a < 3; a++) {
+            real goal = cm[a] + (R[3 * a] * q[0] + R[3 * a + 1] * q[1] + R[3 * a + 2] * q[2]);
+            real corr = (goal - s->x[3 * i + a]) * (real)1.0;
+            s->x[3 * i + a] += corr;
+        }
+    }
+    for (int c = 0; c < 3; c++) s->rb_cm[3 * b + c] = cm[c];
+    for (int k = 0; k < 9; k++) s->rb_R[9 * b + k] = R[k];
+}
+
+/* RigidBodySolver.__init__, :1100-1106: rest centre of mass from the initial
+ * positions (x_0 is the copy of x taken at construction, :1106). */
+static void rigid_init(or_sim *s)
+{
+    for (int b = 0; b < s->nbodies; b++) {
+        rigid_com(s, b, s->rb_rest_cm + 3 * b);
+        for (int c = 0; c < 3; c++) s->rb_cm[3 * b + c] = s->rb_rest_cm[3 * b + c];
+        for (int k = 0; k < 9; k++) s->rb_R[9 * b + k] = (k % 4) == 0 ? (real)1.0 : (real)0.0;
+    }
+}
+
+/* SPHBase.step, :929-931: each body in turn, coupling_interval substeps of
+ * ground collision and shape matching. */
+static void rigid_step(or_sim *s)
+{
+    for (int b = 0; b < s->nbodies; b++)
+        for (int c = 0; c < s->p.coupling_interval; c++) {
+            rigid_ground(s);
+            rigid_project(s, b);
+        }
+}
+
+/* ------------------------------------------------------------------------ */
 /* public API                                                               */
 /* ------------------------------------------------------------------------ */
 or_sim *or_create(const or_params *p, int32_t n, const float *x, const float *v, const int32_t *kind)
@@ -1197,8 +1296,15 @@ or_sim *or_create(const or_params *p, int32_t n, const float *x, const float *v,
     s->gslot = (int32_t *)calloc((size_t)s->ncells * MAX_PIC, sizeof(int32_t));
     s->nbr = (int32_t *)calloc((size_t)n * MAX_NB, sizeof(int32_t));
     s->nnbr = (int32_t *)calloc(n, sizeof(int32_t));
+    s->body = (int32_t *)malloc(sizeof(int32_t) * n);
     for (int i = 0; i < n; i++) {
-        s->kind[i] = kind[i];
+        if (kind[i] < 0 || kind[i] >= OR_KIND_RIGID + OR_MAX_BODIES) {
+            or_destroy(s);
+            return NULL;
+        }
+        s->kind[i] = kind[i] >= OR_KIND_RIGID ? OR_KIND_RIGID : kind[i];
+        s->body[i] = kind[i] >= OR_KIND_RIGID ? kind[i] - OR_KIND_RIGID : -1;
+        if (s->body[i] >= s->nbodies) s->nbodies = s->body[i] + 1;
         if (kind[i] == OR_KIND_FLUID) s->nf++;
         if (kind[i] == OR_KIND_ELASTIC) s->ne++;
         for (int c = 0; c < 3; c++) {
@@ -1210,6 +1316,21 @@ or_sim *or_create(const or_params *p, int32_t n, const float *x, const float *v,
         s->eta[i] = (real)0.0;
         s->s_of[i] = -1;
     }
+    /* every body index below the largest must have particles */
+    int nb = s->nbodies > 0 ? s->nbodies : 1;
+    int32_t *bc = (int32_t *)calloc(nb, sizeof(int32_t));
+    for (int i = 0; i < n; i++)
+        if (s->body[i] >= 0) bc[s->body[i]]++;
+    for (int b = 0; b < s->nbodies; b++)
+        if (bc[b] == 0) {
+            free(bc);
+            or_destroy(s);
+            return NULL;
+        }
+    free(bc);
+    s->rb_rest_cm = (real *)calloc((size_t)3 * nb, sizeof(real));
+    s->rb_cm = (real *)calloc((size_t)3 * nb, sizeof(real));
+    s->rb_R = (real *)calloc((size_t)9 * nb, sizeof(real));
     return s;
 }
 
@@ -1218,18 +1339,21 @@ void or_destroy(or_sim *s)
     if (!s) return;
     void *ptrs[] = {s->x, s->v, s->a, s->x0, s->vol, s->rho, s->alpha, s->dadv, s->rest_vol, s->gam, s->eta,
                     s->wx, s->wr, s->wz, s->wp, s->wq, s->ws, s->wvd, s->wdinv, s->wmu, s->wpair, s->kind, s->s_of, s->gcount, s->gslot,
-                    s->nbr, s->nnbr, s->s_pid, s->rest, s->rest_cnt, s->R, s->stress, s->Fm};
+                    s->nbr, s->nnbr, s->s_pid, s->rest, s->rest_cnt, s->R, s->stress, s->Fm,
+                    s->body, s->rb_rest_cm, s->rb_cm, s->rb_R};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(s);
 }
 
-/* Elasticity.__init__ -> initValues (:1216-1236), then SPHBase.initialize
- * (:957-960): search, static boundary volume, moving boundary volume. */
+/* RigidBodySolver.__init__ (:1100-1106) and Elasticity.__init__ -> initValues
+ * (:1216-1236), then SPHBase.initialize (:957-960): search, static boundary
+ * volume, moving boundary volume. */
 int32_t or_init(or_sim *s)
 {
     s->status = 0;
     or_search(s);
-    elastic_init(s);
+    elastic_init(s); /* x0 = x for every particle */
+    rigid_init(s);
     boundary_volume(s, 0);
     boundary_volume(s, 1);
     s->initialized = 1;
@@ -1309,6 +1433,7 @@ int32_t or_step(or_sim *s, or_stats *st)
         elastic_rot_stress(s);
         elastic_forces(s);
     }
+    if (s->nbodies > 0) rigid_step(s);
     advect(s);
     clamp_boundary(s);
     /* pairs of the lists a step uses: fluid and elastic particles (the static
@@ -1354,6 +1479,18 @@ int32_t or_get(const or_sim *s, int32_t field, double *out)
         break;
     default: return -1;
     }
+    return 0;
+}
+
+int32_t or_rigid_frame(const or_sim *s, int32_t b, double rest_cm[3], double cm[3], double R[9])
+{
+    if (b < 0 || b >= s->nbodies) return -1;
+    for (int c = 0; c < 3; c++) {
+        if (rest_cm) rest_cm[c] = (double)s->rb_rest_cm[3 * b + c];
+        if (cm) cm[c] = (double)s->rb_cm[3 * b + c];
+    }
+    if (R)
+        for (int k = 0; k < 9; k++) R[k] = (double)s->rb_R[9 * b + k];
     return 0;
 }
 

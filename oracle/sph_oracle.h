@@ -59,7 +59,7 @@ typedef struct or_params {
     double poisson_ratio;
     double particle_mass;         /* <= 0: volume0 * 1000 (:743) */
     int32_t warm_start;           /* bit 0: Weiler CG starts from v + (x - v) of the previous solve */
-    int32_t reserved;
+    int32_t coupling_interval;    /* rigid-body substeps per step, Parameter.coupling_interval (:616) */
 } or_params;
 
 typedef struct or_stats {
@@ -74,10 +74,12 @@ typedef struct or_stats {
     int64_t num_pairs;
 } or_stats;
 
-/* particle kinds */
+/* particle kinds; kind OR_KIND_RIGID + b is a particle of rigid body b */
 #define OR_KIND_FLUID 0
 #define OR_KIND_STATIC 1
 #define OR_KIND_ELASTIC 2
+#define OR_KIND_RIGID 3
+#define OR_MAX_BODIES 4096
 
 /* status bits */
 #define OR_ST_NEIGHBOR_OVERFLOW 1
@@ -102,6 +104,10 @@ int32_t or_init(or_sim *s);
 int32_t or_search(or_sim *s);
 int32_t or_step(or_sim *s, or_stats *st);
 int32_t or_get(const or_sim *s, int32_t field, double *out);
+/* Rigid body b: rest centre of mass, and the centre of mass and rotation of
+ * its last shape-matching projection (identity before the first step).
+ * Returns -1 if b is not a body of the scene. */
+int32_t or_rigid_frame(const or_sim *s, int32_t b, double rest_cm[3], double cm[3], double R[9]);
 int32_t or_neighbors(const or_sim *s, int32_t *nbr, int32_t *cnt);
 int32_t or_set(or_sim *s, int32_t field, const double *in);
 int32_t or_num_threads(void);

@@ -220,7 +220,7 @@ def test_upload_field_then_search():
     ("cell_overflow", L.NN_E_CELL_OVERFLOW),
     ("neighbor_overflow", L.NN_E_NEIGHBOR_OVERFLOW),
     ("out_of_grid", L.NN_E_OUT_OF_GRID),
-    ("rigid", L.NN_E_UNSUPPORTED),
+    ("bad_kind", L.NN_E_ARG),
 ])
 def test_errors_are_reported(case, code):
     p = SceneParams(domain_end=(0.4, 0.4, 0.4))
@@ -234,7 +234,7 @@ def test_errors_are_reported(case, code):
         pts = np.array([[-0.05, 0.1, 0.1], [0.1, 0.1, 0.1]])
     else:
         pts = np.array([[0.1, 0.1, 0.1]])
-        kind = np.array([3], np.int32)
+        kind = np.array([-1], np.int32)
     ctx = L.Context(p, len(pts))
     with pytest.raises(L.NNError) as e:
         ctx.upload(pts, np.zeros(len(pts), np.int32) if kind is None else kind)

@@ -59,11 +59,12 @@ def alg_bytes(kernel, N, Nf, Ne, E, Ef, warm=0):
         "press_update": 5 * N + 9 * Nf + Ef,
         "press_source": 8 * N + 3 * Nf + Ef,    # + rho in
         "nonpressure": 14 * N + 2 * Nf + Ef,    # x,V,v,kind, rho_j; acc and v' out
-        "cg_matvec": 5 * N + 10 * Nf + 2 * Ef,  # x,V,kind; u, rho, count in; w out; list + pair scalar
-        "cg_update": N + 50 * Nf,               # u, w, p, s, x, r, M in; p, s, x, r, u out
-        "cg_init": (11 + 4 * warm) * N + (11 + 4 * warm) * Nf + 2 * Ef,  # x,V,v,mu,rho,kind (+e); count;
-                                                # M, r0 (+Ae) out; list in, pair scalar out
-        "cg_guess": N + (26 + 8 * warm) * Nf,   # v, r0, M (+e, Ae) in; x, r, u out
+        # the CG's 3-vectors are 3 words (u, w, p, s, x, r, e; M is 6)
+        "cg_matvec": 4 * N + 8 * Nf + 2 * Ef,   # x, kind; u, rho, count in; w out; list + pair scalar
+        "cg_update": N + 39 * Nf,               # kind; u, w, p, s, x, r, M in; p, s, x, r, u out
+        "cg_init": (13 + 3 * warm) * N + (10 + 3 * warm) * Nf + 2 * Ef,  # x,V,v,mu,rho,kind (+e) in,
+                                                # x out; count in, M, r0 (+Ae) out; list in, pair scalar out
+        "cg_guess": N + (21 + 6 * warm) * Nf,   # kind; v, r0, M (+e, Ae) in; x, r, u out
         "strain": 8 * N + 3 * Nf + Ef,
         "advect": 12 * N,
         "clear_acc": 5 * N,

@@ -215,10 +215,11 @@ void rigid_rest_centres(SceneHost& s, const float* x) {
   }
 }
 
-// bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wx,wr,wu,ww,wp,ws,wd0,x0,vdiff (14 float4),
-// wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int), rho,alpha,src,mu,gam,eta (6 float),
-// nbr and spair (60 int + 60 float), partials (16 doubles per 64 slots), wave table
-static constexpr long long BYTES_PER_SLOT = 14 * 16 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 3;
+// bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wd0,x0,vdiff,vdiff_alt (9 float4),
+// wx,wr,wu,ww,wp,ws,pos3 (7 float3), wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int),
+// rho,alpha,src,mu,gam,eta (6 float), nbr and spair (60 int + 60 float), partials (16 doubles per 64
+// slots), wave table
+static constexpr long long BYTES_PER_SLOT = 9 * 16 + 7 * 12 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 3;
 static constexpr long long BYTES_PER_MEMBER = 4 + 4 + MAX_REST * 4 + 4 * 16 + 1;
 
 long long scene_footprint(long long n, long long ns, long long cells) {
@@ -263,7 +264,7 @@ void Batch::swap_sort_buffers() {
   std::swap(d.pos, d.pos_alt);
   std::swap(d.vel, d.vel_alt);
   std::swap(d.info, d.info_alt);
-  std::swap(d.vdiff, d.ws);  // k_place moved the guesses into ws (free outside the solve)
+  std::swap(d.vdiff, d.vdiff_alt);
 }
 
 // Per-scene fields that depend on the scenes' order and offsets (p_begin,
@@ -507,6 +508,8 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.ww, Pc);
   A(nd.wp, Pc);
   A(nd.ws, Pc);
+  A(nd.pos3, Pc);
+  A(nd.vdiff_alt, Pc);
   A(nd.wd0, Pc);
   A(nd.wd1, Pc);
   A(nd.spair, (size_t)Pc * MAX_NB);

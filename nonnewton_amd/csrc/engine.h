@@ -147,17 +147,20 @@ struct Dev {
   int* cg_list;           // 2 x W: waves of the scenes still iterating, by iteration parity
   int* cg_list_n;         // 2 counts
   int* n_active;     // [0] divergence, [1] pressure, [2] cg
-  // Weiler CG
-  float4* wx;  // iterate x
-  float4* wr;  // residual r
-  float4* wu;  // preconditioned residual u = M r
-  float4* ww;  // w = A u
-  float4* wp;  // direction p
-  float4* ws;  // s = A p
+  // Weiler CG: 3-vectors packed at 12 bytes (the sweeps stream them and the
+  // matvec gathers u and the position of every neighbor)
+  float3* wx;  // iterate x
+  float3* wr;  // residual r
+  float3* wu;  // preconditioned residual u = M r
+  float3* ww;  // w = A u
+  float3* wp;  // direction p
+  float3* ws;  // s = A p
+  float3* pos3;  // positions during the solve (written by cg_init for every particle of a Weiler scene)
   float4* wd0;  // inverse diagonal block: (d00, d01, d02, d11)
   float2* wd1;  //                          (d12, d22)
   float* spair; // per-pair scalar s_ij of the Weiler operator, at the list word of the pair (nb_word)
   float4* vdiff; // x - v of the previous Weiler solve (initial guess; moves with the particle)
+  float4* vdiff_alt;  // its sort buffer
   float* mu;    // dynamic viscosity per particle
   float* gam;   // strain-rate norm
   float* eta;   // kinematic viscosity from the model

@@ -286,7 +286,7 @@ __global__ void k_nonpressure(Dev d) {
       ay += wvy;
       az += wvz;
     } else if (weiler) {
-      const float4 x = d.wx[i];
+      const float3 x = d.wx[i];
       const float dx = x.x - vi.x, dy = x.y - vi.y, dz = x.z - vi.z;
       ax += dx / sc.dt;
       ay += dy / sc.dt;

@@ -197,7 +197,7 @@ __global__ void k_place(Dev d) {
   d.pos_alt[dest] = d.pos[i];
   d.vel_alt[dest] = d.vel[i];
   d.info_alt[dest] = inf;
-  if (sc.flags & SF_WARM_CG) d.ws[dest] = d.vdiff[i];
+  if (sc.flags & SF_WARM_CG) d.vdiff_alt[dest] = d.vdiff[i];
   d.inv[sc.p_begin + my] = dest - sc.p_begin;
 }
 

@@ -163,6 +163,11 @@ __global__ void k_cg_init(Dev d) {
   m3 = quad_sum(m3);
   m4 = quad_sum(m4);
   m5 = quad_sum(m5);
+  // the positions the CG gathers, for every particle of the scene
+  if (q == 0 && i < sc.p_begin + sc.n) {
+    const float4 p = valid ? xi : d.pos[i];
+    d.pos3[i] = make_float3(p.x, p.y, p.z);
+  }
   if (valid && q == 0) {
     const float f = sc.dt / d.rho[i];
     const float qx = vi.x - f * ax, qy = vi.y - f * ay, qz = vi.z - f * az;
@@ -184,10 +189,10 @@ __global__ void k_cg_init(Dev d) {
     }
     d.wd0[i] = w0;
     d.wd1[i] = w1;
-    d.wr[i] = make_float4(rrx, rry, rrz, 0.f);
+    d.wr[i] = make_float3(rrx, rry, rrz);
     if (warm) {
       const float zx = ei.x - f * bx, zy = ei.y - f * by, zz = ei.z - f * bz;
-      d.ww[i] = make_float4(zx, zy, zz, 0.f);
+      d.ww[i] = make_float3(zx, zy, zz);
       v[0] = (double)ei.x * (double)rrx + (double)ei.y * (double)rry + (double)ei.z * (double)rrz;
       v[1] = (double)ei.x * (double)zx + (double)ei.y * (double)zy + (double)ei.z * (double)zz;
     }
@@ -205,14 +210,14 @@ __global__ void k_cg_guess(Dev d) {
   double v[3] = {0.0, 0.0, 0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
     const float4 b = d.vel[i];
-    float4 r = d.wr[i];
-    float4 x = make_float4(b.x, b.y, b.z, 0.f);
+    float3 r = d.wr[i];
+    float3 x = make_float3(b.x, b.y, b.z);
     if (sc.flags & SF_WARM_CG) {
       const float th = (float)d.state[s].cg_theta;
       const float4 e = d.vdiff[i];
-      const float4 z = d.ww[i];
-      x = make_float4(b.x + th * e.x, b.y + th * e.y, b.z + th * e.z, 0.f);
-      r = make_float4(r.x - th * z.x, r.y - th * z.y, r.z - th * z.z, 0.f);
+      const float3 z = d.ww[i];
+      x = make_float3(b.x + th * e.x, b.y + th * e.y, b.z + th * e.z);
+      r = make_float3(r.x - th * z.x, r.y - th * z.y, r.z - th * z.z);
     }
     const float4 w0 = d.wd0[i];
     const float2 w1 = d.wd1[i];
@@ -221,7 +226,7 @@ __global__ void k_cg_guess(Dev d) {
     const float uz = w0.z * r.x + w1.x * r.y + w1.y * r.z;
     d.wx[i] = x;
     d.wr[i] = r;
-    d.wu[i] = make_float4(ux, uy, uz, 0.f);
+    d.wu[i] = make_float3(ux, uy, uz);
     v[0] = (double)b.x * (double)b.x + (double)b.y * (double)b.y + (double)b.z * (double)b.z;
     v[1] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[2] = (double)r.x * (double)ux + (double)r.y * (double)uy + (double)r.z * (double)uz;
@@ -231,10 +236,12 @@ __global__ void k_cg_guess(Dev d) {
 
 // w = A u (u gathered from the neighbors); partial u.w into partial array 3
 // (start: after cg_guess, every Weiler scene; read by the start reduction --
-// otherwise: an iteration, active scenes; read by the next update).  D list positions of a lane per batch: their list words and pair
-// scalars, then their neighbors' positions and u, are loaded before any of
-// them is used (a shorter chain of dependent loads); the terms are added in
-// list order whatever D is.
+// otherwise: an iteration, active scenes; read by the next update).  D list
+// positions of a lane per batch: their list words and pair scalars, then their
+// neighbors' positions (pos3) and u, are loaded before any of them is used (a
+// shorter chain of dependent loads); the terms are added in list order
+// whatever D is.  A pair with s_ij = 0 -- every neighbor that is not fluid --
+// adds nothing and is skipped, so the neighbor's u is never read there.
 template <int D>
 __global__ void k_cg_matvec(Dev d, int start) {
   int i, q, s, hw;
@@ -245,16 +252,16 @@ __global__ void k_cg_matvec(Dev d, int start) {
   double v[1] = {0.0};
   const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
   float ax = 0.f, ay = 0.f, az = 0.f;
-  float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
+  float3 ui = make_float3(0.f, 0.f, 0.f);
   if (valid) {
-    const float4 xi = d.pos[i];
+    const float3 xi = d.pos3[i];
     ui = d.wu[i];
     const int cnt = d.nnbr[i];
     const long P = d.P;
     for (int k = q; k < cnt; k += 4 * D) {
       int j[D];
       float sp[D];
-      float4 xj[D], uj[D];
+      float3 xj[D], uj[D];
 #pragma unroll
       for (int e = 0; e < D; e++) {
         const bool in = k + 4 * e < cnt;
@@ -263,12 +270,12 @@ __global__ void k_cg_matvec(Dev d, int start) {
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
-        xj[e] = d.pos[j[e]];
+        xj[e] = d.pos3[j[e]];
         uj[e] = d.wu[j[e]];
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
-        if (k + 4 * e < cnt && xj[e].w > 0.0f) {
+        if (sp[e] != 0.0f) {
           const float rx = xi.x - xj[e].x, ry = xi.y - xj[e].y, rz = xi.z - xj[e].z;
           const float t = sp[e] * ((ui.x - uj[e].x) * rx + (ui.y - uj[e].y) * ry + (ui.z - uj[e].z) * rz);
           ax += t * rx;
@@ -284,7 +291,7 @@ __global__ void k_cg_matvec(Dev d, int start) {
   if (valid && q == 0) {
     const float f = sc.dt / d.rho[i];
     const float wx = ui.x - f * ax, wy = ui.y - f * ay, wz = ui.z - f * az;
-    d.ww[i] = make_float4(wx, wy, wz, 0.f);
+    d.ww[i] = make_float3(wx, wy, wz);
     v[0] = (double)ui.x * (double)wx + (double)ui.y * (double)wy + (double)ui.z * (double)wz;
   }
   wave_partials<1>(d, hw, v, 3);
@@ -309,16 +316,16 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
     const int i = w * 64 + (threadIdx.x >> 2);
     const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
+    float3 ui = make_float3(0.f, 0.f, 0.f);
     if (valid) {
-      const float4 xi = d.pos[i];
+      const float3 xi = d.pos3[i];
       ui = d.wu[i];
       const int cnt = d.nnbr[i];
       const long P = d.P;
       for (int kk = q; kk < cnt; kk += 4 * D) {
         int j[D];
         float sp[D];
-        float4 xj[D], uj[D];
+        float3 xj[D], uj[D];
 #pragma unroll
         for (int e = 0; e < D; e++) {
           const bool in = kk + 4 * e < cnt;
@@ -327,12 +334,12 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {
-          xj[e] = d.pos[j[e]];
+          xj[e] = d.pos3[j[e]];
           uj[e] = d.wu[j[e]];
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {
-          if (kk + 4 * e < cnt && xj[e].w > 0.0f) {
+          if (sp[e] != 0.0f) {
             const float rx = xi.x - xj[e].x, ry = xi.y - xj[e].y, rz = xi.z - xj[e].z;
             const float t = sp[e] * ((ui.x - uj[e].x) * rx + (ui.y - uj[e].y) * ry + (ui.z - uj[e].z) * rz);
             ax += t * rx;
@@ -349,7 +356,7 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
     if (valid && q == 0) {
       const float f = sc.dt / d.rho[i];
       const float wx = ui.x - f * ax, wy = ui.y - f * ay, wz = ui.z - f * az;
-      d.ww[i] = make_float4(wx, wy, wz, 0.f);
+      d.ww[i] = make_float3(wx, wy, wz);
       v[0] = (double)ui.x * (double)wx + (double)ui.y * (double)wy + (double)ui.z * (double)wz;
     }
     wave_partials<1>(d, 4 * w + (threadIdx.x >> 6), v, 3);  // the quarter-wave index k_cg_matvec uses
@@ -454,20 +461,20 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
   const int i = w * 64 + (t & 63);
   double v[2] = {0.0, 0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
-    const float4 u = d.wu[i];
-    const float4 ww = d.ww[i];
-    float4 p, sv;
+    const float3 u = d.wu[i];
+    const float3 ww = d.ww[i];
+    float3 p, sv;
     if (first) {
       p = u;
       sv = ww;
     } else {
-      const float4 po = d.wp[i];
-      const float4 so = d.ws[i];
-      p = make_float4(u.x + bef * po.x, u.y + bef * po.y, u.z + bef * po.z, 0.f);
-      sv = make_float4(ww.x + bef * so.x, ww.y + bef * so.y, ww.z + bef * so.z, 0.f);
+      const float3 po = d.wp[i];
+      const float3 so = d.ws[i];
+      p = make_float3(u.x + bef * po.x, u.y + bef * po.y, u.z + bef * po.z);
+      sv = make_float3(ww.x + bef * so.x, ww.y + bef * so.y, ww.z + bef * so.z);
     }
-    float4 x = d.wx[i];
-    float4 r = d.wr[i];
+    float3 x = d.wx[i];
+    float3 r = d.wr[i];
     x.x += alf * p.x;
     x.y += alf * p.y;
     x.z += alf * p.z;
@@ -483,7 +490,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
     d.ws[i] = sv;
     d.wx[i] = x;
     d.wr[i] = r;
-    d.wu[i] = make_float4(zx, zy, zz, 0.f);
+    d.wu[i] = make_float3(zx, zy, zz);
     v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[1] = (double)r.x * (double)zx + (double)r.y * (double)zy + (double)r.z * (double)zz;
   }

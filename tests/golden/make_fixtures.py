@@ -27,7 +27,7 @@ def reference_data():
     for name in ("test_neighbor", "onepar", "test_deformable"):
         pts = ply.read_ply_particles(os.path.join(REF, "models", name + ".ply"))
         np.save(os.path.join(HERE, f"{name}_points.npy"), pts)
-    for name in ("test_neighbor", "test_nonNewton", "test_deformable", "dambreak"):
+    for name in ("test_neighbor", "test_nonNewton", "test_deformable", "dambreak", "rigidbody", "test_oneway_coupling"):
         shutil.copyfile(os.path.join(REF, "scenes", name + ".json"), os.path.join(HERE, f"scene_{name}.json"))
 
 

@@ -57,3 +57,32 @@ def test_scene_json_run(tmp_path):
     assert np.abs(R - np.eye(3)).max() < 1e-3
     x = meta.pd.x.to_numpy()
     assert np.all(x[:, 1] < pts[:, 1].max() + 1e-6)  # falling
+
+
+def test_rigid_coupling_scene_json_run(tmp_path):
+    """main() on the reference's test_oneway_coupling.json (a fluid cube above a
+    RIGID cube) with lattice stand-ins for its cube PLY files: one rigid body in
+    meta.rbs, kinds in the reference's phase order, and the positions after 5
+    steps equal to the oracle's bit for bit."""
+    root = tmp_path / "SPH"
+    (root / "data" / "models").mkdir(parents=True)
+    (root / "data" / "scenes").mkdir(parents=True)
+    fluid = scenes.lattice((0.26, 1.26, 0.26), (12, 12, 12)).astype(np.float32)
+    body = scenes.lattice((0.26, 0.26, 0.26), (12, 12, 12)).astype(np.float32)
+    ply.export_ply_particles(fluid, str(root / "data" / "models" / "cube2.ply"))
+    ply.export_ply_particles(body, str(root / "data" / "models" / "cube.ply"))
+    cfg_path = root / "data" / "scenes" / "c.json"
+    shutil.copyfile(os.path.join(GOLDEN, "scene_test_oneway_coupling.json"), cfg_path)
+    sph.main(str(cfg_path), max_step=5)
+    assert len(meta.rbs) == 1 and meta.rbs[0].body == 0
+    n_f = len(fluid)
+    assert np.array_equal(meta.pd.kind, np.r_[np.zeros(n_f), np.full(len(body), 3)].astype(np.int32))
+    with open(cfg_path) as f:
+        params, _ = sph.params_from_config(json.load(f))
+    assert params.coupling_interval == 1
+    orc = O.OracleSim(params, np.concatenate([fluid, body]), meta.pd.kind)
+    orc.init()
+    for _ in range(5):
+        orc.step()
+    assert np.array_equal(meta.pd.x.to_numpy().astype(np.float64), orc.get(O.F_X))
+    assert np.array_equal(meta.pd.v.to_numpy().astype(np.float64), orc.get(O.F_V))

@@ -136,7 +136,7 @@ def _admit_all(scs, block):
     return pool, slots
 
 
-def run_batched(scs, steps, warmup, profile=True, block=0, tune=True):
+def run_batched(scs, steps, warmup, profile=True, block=0, tune=True, kernel=None):
     """The batched form.  Launch shapes are tuned on a separate pool holding the
     same scenes (the tuner advances its scenes), so the timed pool starts from
     the same state whether tuned or not."""
@@ -164,7 +164,7 @@ def run_batched(scs, steps, warmup, profile=True, block=0, tune=True):
         pool.profile(True)
         pool.tick()
         breakdown = pool.kernel_times()
-        dominant = max(breakdown.items(), key=lambda kv: kv[1][0])[0]
+        dominant = kernel or max(breakdown.items(), key=lambda kv: kv[1][0])[0]
         pool.profile(True, kernels=[dominant], stride=EVENT_STRIDE)
     lat = []
     raw = []
@@ -178,12 +178,14 @@ def run_batched(scs, steps, warmup, profile=True, block=0, tune=True):
     names = raw[0].dtype.names if raw else ()
     per_step_stats = [[{f: a[f][k].item() for f in names} for k in range(len(slots))] for a in raw]
     times = {}
+    launches = {}
     if profile:
         ms, timed = pool.kernel_times()[dominant]
-        times = {dominant: (ms, timed, pool.kernel_launches()[dominant])}
+        launches = pool.kernel_launches()  # every kernel's launches in the timed region
+        times = {dominant: (ms, timed, launches[dominant])}
     pool.profile(False)
     pool.close()
-    return elapsed, lat, per_step_stats, times, breakdown, shapes
+    return elapsed, lat, per_step_stats, times, breakdown, shapes, launches
 
 
 def run_plain(scs, steps, warmup, block=0, shapes=None):
@@ -307,6 +309,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--roofline-kernel", default=None, metavar="KERNEL",
+                    help="report the roofline of KERNEL instead of the one that dominates the probe tick")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (calibrates the rocprofv3 FETCH_SIZE and "
                          "WRITE_SIZE counters against a kernel of known traffic) and exit")
@@ -324,8 +328,8 @@ def main():
     n_all = sum(s.n for s in scs)
     log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
 
-    elapsed, lat, stats, ktimes, breakdown, shapes = run_batched(scs, args.steps, args.warmup, block=args.block,
-                                                                 tune=not args.no_tune)
+    elapsed, lat, stats, ktimes, breakdown, shapes, launch_counts = run_batched(
+        scs, args.steps, args.warmup, block=args.block, tune=not args.no_tune, kernel=args.roofline_kernel)
     value = n_dyn * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] batched: {ms_per_step:.3f} ms/tick, {value:.4g} particle-steps/s")
@@ -361,6 +365,16 @@ def main():
                     "launches": launches, "timed_launches": timed, "avg_launch_us": round(1e6 * avg_s, 2),
                     "bytes_per_launch": round(per_launch),
                     "share_of_timed_region": round(avg_s * launches / elapsed, 3)}
+        # algorithmic bytes per launch of the other CG kernels over the same ticks
+        # (the rocprofv3 traffic summaries compare their counters with these)
+        per_kernel = {}
+        for k in ("cg_matvec", "cg_update"):
+            n_l = launch_counts.get(k, 0)
+            tb = sum(scene_algorithmic_bytes(k, sc, st) or 0 for ss in stats for sc, st in zip(scs, ss))
+            if n_l:
+                per_kernel[k] = round(tb / n_l)
+        if roof is not None:
+            roof["alg_bytes_per_launch"] = per_kernel
         log("[bench] kernel times of one tick (ms, launches): " +
             ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0])
                       if v[1]))

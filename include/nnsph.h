@@ -223,8 +223,8 @@ int32_t nn_set_kernel_depth(nn_ctx *ctx, int32_t kernel, int32_t depth);
  * stride-th launch, so that the other launches run without events between
  * them.  nn_*_kernel_times copies the accumulated milliseconds and the number
  * of timed launches of kernel ids 0..n-1 (names from nn_kernel_name,
- * NN_KERNEL_COUNT ids); nn_*_kernel_launches the number of launches made while
- * timing was on, timed or not. */
+ * NN_KERNEL_COUNT ids); nn_*_kernel_launches the number of launches of every
+ * kernel made while timing was on, timed or not, masked or not. */
 #define NN_KERNEL_COUNT 21
 const char *nn_kernel_name(int32_t id);
 int32_t nn_profile(nn_ctx *ctx, int32_t enable);

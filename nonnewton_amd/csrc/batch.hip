@@ -716,10 +716,11 @@ KernelTimer::~KernelTimer() {
 }
 
 int KernelTimer::begin(int id, hipStream_t st) {
-  open = on && ((mask >> id) & 1u);
-  if (!open) return NN_OK;
-  open = (seen[id]++ % stride) == 0;
-  if (!open) return NN_OK;
+  open = false;
+  if (!on) return NN_OK;
+  const long long n = seen[id]++;  // every kernel's launches are counted while on
+  if (!((mask >> id) & 1u) || n % stride != 0) return NN_OK;
+  open = true;
   while (events.size() < used + 2) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));

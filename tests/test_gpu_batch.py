@@ -153,6 +153,7 @@ def test_sampled_kernel_timing():
     ms, timed = times["cg_matvec"]
     assert seen["cg_matvec"] >= 6 and timed == (seen["cg_matvec"] + 2) // 3 and ms > 0
     assert all(n == 0 for k, (_, n) in times.items() if k != "cg_matvec")
+    assert seen["cg_update"] > 0 and seen["advect"] == 3  # launches of kernels not timed are counted too
     pool.profile(False)
     pool.close()
 

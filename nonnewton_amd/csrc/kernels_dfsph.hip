@@ -242,8 +242,13 @@ __global__ void k_nonpressure(Dev d) {
         float4 x, v;
         float rho;
       };
+      // surface tension reads positions only: the velocities and densities
+      // are gathered only for a viscosity term formed here
+      const bool need_v = expl || bnu_on;
+      const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
       for_neighbors<B>(
-          d, i, cnt, [&](int j) { return NJ{d.pos[j], d.vel[j], d.rho[j]}; },
+          d, i, cnt,
+          [&](int j) { return need_v ? NJ{d.pos[j], d.vel[j], d.rho[j]} : NJ{d.pos[j], zero4, 0.0f}; },
           [&](int, const NJ& nj) {
             const float4 xj = nj.x, vj = nj.v;
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;

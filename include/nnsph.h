@@ -236,6 +236,62 @@ int32_t nn_pool_profile_mask(nn_pool *pool, uint32_t mask, int32_t stride);
 int32_t nn_pool_kernel_times(nn_pool *pool, double *ms, int64_t *launches, int32_t n);
 int32_t nn_pool_kernel_launches(nn_pool *pool, int64_t *seen, int32_t n);
 
+/* ------------------------------------------------------------------------
+ * Surface reconstruction: the particle-to-mesh path of SRTool
+ * (SPH/surf_recon/srtool.py), a dual-contouring mesh of the particles'
+ * anisotropic smooth field.
+ *
+ * The field is accumulated in 2^-32 fixed point, so a run's results do not
+ * depend on the order the GPU adds the particles' contributions; mesh vertices
+ * are numbered in grid order of their cubes (x slowest, z fastest), triangles
+ * in grid order of the cube that emits them.  The library allocates device
+ * memory in nn_surface_run* when the grid grows.
+ * ------------------------------------------------------------------------ */
+typedef struct nn_surface nn_surface;
+
+typedef struct nn_surface_params {
+  double origin[3];          /* bounding_box_extent[0] (SRTool.__init__, srtool.py:887-899) */
+  double voxel_size;         /* voxel_size (:889) */
+  double particle_radius;    /* particle_radius (:890): volume = 4/3 pi r^3 (:945) */
+  double smooth_radius;      /* dual_contouring(smooth_radius) (:1264-1271) */
+  double isolevel;           /* dual_contouring(isolevel) */
+  int32_t anisotropic;       /* use_smooth_fluid_field (default 1): add the anisotropic field */
+  int32_t smooth_iterations; /* smooth_sdf(iteration) (:1425), 1 in dual_contouring (:1289); 0..64 */
+  int32_t normals;           /* record_normals (:891): per-vertex normals (process_normal, :1384-1410) */
+  int32_t reserved;
+} nn_surface_params;
+
+/* SRTool(...) (srtool.py:887-991).  NN_E_UNSUPPORTED when the anisotropic box
+ * of 4 * smooth_radius / voxel_size + 2 vertices exceeds 32 per axis. */
+int32_t nn_surface_create(const nn_surface_params *params, nn_surface **out);
+int32_t nn_surface_destroy(nn_surface *s);
+/* Cap on the device bytes a run's grids may take (0 = none): NN_E_OOM above it. */
+int32_t nn_surface_set_budget(nn_surface *s, int64_t bytes);
+/* SRTool.dual_contouring(isolevel, smooth_radius, num_particles, pos)
+ * (srtool.py:1264-1304) on n host positions (x, y, z float triples).
+ * NN_E_OUT_OF_GRID: a position lies below the box origin's voxel or is not
+ * finite; NN_E_CELL_OVERFLOW: more than 64 particles in a voxel
+ * (max_num_particle_per_voxel, :685); NN_E_NAN: a field contribution is not
+ * finite or exceeds 2^20. */
+int32_t nn_surface_run(nn_surface *s, const float *pos, int64_t n);
+/* The same on positions already in device memory (n x 3 float, caller order). */
+int32_t nn_surface_run_device(nn_surface *s, const float *device_pos, int64_t n);
+/* num_vertices / num_indices of the last run and its grid (srtool.py:917-918):
+ * out[0..2] first grid vertex, out[3..5] grid dims (vertices), out[6] vertices,
+ * out[7] triangles, out[8] written 32^3 blocks, out[9..11] first voxel of the
+ * particle hash, out[12..14] its dims; *sd = the smoothing's sd (:1437-1456). */
+int32_t nn_surface_info(nn_surface *s, int64_t *out, double *sd);
+/* mesh_vertex, mesh_index as triangles, mesh_normal (export_mesh, :1802-1838);
+ * normals may be NULL, and must be NULL unless params.normals was set. */
+int32_t nn_surface_mesh(nn_surface *s, float *vertices, int32_t *faces, float *normals);
+/* voxel_vertex_value over the grid (dims from nn_surface_info, z fastest):
+ * which 0 = after the two field passes, 1 = after smoothing (what the
+ * contouring reads); 0 outside written blocks. */
+int32_t nn_surface_field(nn_surface *s, int32_t which, float *out, size_t count);
+/* G (row-major 3x3) and x_bar per particle of the last anisotropic run
+ * (pre_process_data, srtool.py:1007-1112). */
+int32_t nn_surface_aniso(nn_surface *s, float *G, float *xbar, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,6 +88,22 @@ class NnStepStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class NnSurfaceParams(ctypes.Structure):
+    """nn_surface_params of include/nnsph.h."""
+
+    _fields_ = [
+        ("origin", ctypes.c_double * 3),
+        ("voxel_size", ctypes.c_double),
+        ("particle_radius", ctypes.c_double),
+        ("smooth_radius", ctypes.c_double),
+        ("isolevel", ctypes.c_double),
+        ("anisotropic", ctypes.c_int32),
+        ("smooth_iterations", ctypes.c_int32),
+        ("normals", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class NNError(RuntimeError):
     def __init__(self, code: int, where: str, text: str = ""):
         self.code = code
@@ -157,6 +173,15 @@ def load():
         "nn_pool_profile_mask": (i32, [vp, ctypes.c_uint32, i32]),
         "nn_pool_kernel_launches": (i32, [vp, P(i64), i32]),
         "nn_pool_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
+        "nn_surface_create": (i32, [P(NnSurfaceParams), P(vp)]),
+        "nn_surface_destroy": (i32, [vp]),
+        "nn_surface_set_budget": (i32, [vp, i64]),
+        "nn_surface_run": (i32, [vp, P(ctypes.c_float), i64]),
+        "nn_surface_run_device": (i32, [vp, vp, i64]),
+        "nn_surface_info": (i32, [vp, P(i64), P(ctypes.c_double)]),
+        "nn_surface_mesh": (i32, [vp, P(ctypes.c_float), P(i32), P(ctypes.c_float)]),
+        "nn_surface_field": (i32, [vp, i32, P(ctypes.c_float), sz]),
+        "nn_surface_aniso": (i32, [vp, P(ctypes.c_float), P(ctypes.c_float), i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -103,3 +103,31 @@ def export_ply_particles(particles: np.ndarray, filename: str) -> None:
     with open(filename, "wb") as f:
         f.write(header.encode("ascii"))
         f.write(p.tobytes())
+
+
+def export_ply_mesh(vertices: np.ndarray, faces: np.ndarray, filename: str, normals=None) -> None:
+    """Binary little-endian PLY mesh: vertex x, y, z (and nx, ny, nz) as f4, faces
+    as `list uchar int vertex_indices` (SRTool.export_mesh, SPH/surf_recon/srtool.py:1802-1838)."""
+    v = np.ascontiguousarray(vertices, dtype="<f4").reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype="<i4").reshape(-1, 3)
+    props = "property float x\nproperty float y\nproperty float z\n"
+    if normals is not None:
+        n = np.ascontiguousarray(normals, dtype="<f4").reshape(-1, 3)
+        if n.shape != v.shape:
+            raise ValueError("normals must have one entry per vertex")
+        v = np.concatenate([v, n], axis=1)
+        props += "property float nx\nproperty float ny\nproperty float nz\n"
+    header = (
+        f"ply\nformat binary_little_endian 1.0\nelement vertex {v.shape[0]}\n{props}"
+        f"element face {f.shape[0]}\nproperty list uchar int vertex_indices\nend_header\n"
+    )
+    rec = np.empty(f.shape[0], dtype=[("n", "u1"), ("i", "<i4", (3,))])
+    rec["n"] = 3
+    rec["i"] = f
+    d = os.path.dirname(filename)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    with open(filename, "wb") as out:
+        out.write(header.encode("ascii"))
+        out.write(np.ascontiguousarray(v, dtype="<f4").tobytes())
+        out.write(rec.tobytes())

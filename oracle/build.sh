@@ -12,3 +12,5 @@ $CC $CFLAGS sph_oracle.c -o liboracle_f32.so.tmp -lm
 mv -f liboracle_f32.so.tmp liboracle_f32.so
 $CC $CFLAGS -DORACLE_F64 sph_oracle.c -o liboracle_f64.so.tmp -lm
 mv -f liboracle_f64.so.tmp liboracle_f64.so
+$CC $CFLAGS surface_oracle.c -o liboracle_surface.so.tmp -lm
+mv -f liboracle_surface.so.tmp liboracle_surface.so

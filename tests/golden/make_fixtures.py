@@ -24,7 +24,7 @@ from oracle import oracle as O  # noqa: E402
 
 
 def reference_data():
-    for name in ("test_neighbor", "onepar", "test_deformable"):
+    for name in ("test_neighbor", "onepar", "test_deformable", "torus"):
         pts = ply.read_ply_particles(os.path.join(REF, "models", name + ".ply"))
         np.save(os.path.join(HERE, f"{name}_points.npy"), pts)
     for name in ("test_neighbor", "test_nonNewton", "test_deformable", "dambreak", "rigidbody", "test_oneway_coupling"):

@@ -1,0 +1,340 @@
+"""The surface-reconstruction oracle (oracle/surface_oracle.c, the restatement of
+SRTool.dual_contouring, SPH/surf_recon/srtool.py:1264-1304), pinned by what the
+mathematics and the reference's own example fix, not by retyping its formulas:
+
+  * its exp against numpy's, to 2 ulp;
+  * the QEF solve against least squares (numpy, FP64) for full-rank rows, the
+    minimum-norm solution for rank-deficient ones, a corner of three planes
+    exactly, and the clamped pseudo-inverse for ill-conditioned rows;
+  * one particle's field: the poly6 values in closed form on its 27 vertices,
+    and the anisotropic box with the multiplicities counted by brute force over
+    the reference's voxel x corner loop (a closed form would restate the
+    oracle's own count), isolated particles' G = 2 / h I and x_bar = x;
+  * a lattice particle's G from the covariance formed in FP64 by numpy;
+  * the written blocks, the smoothing's sd and one bilateral pass in FP64;
+  * meshes: the reference's torus example (SPH/surf_recon/example.py, the
+    torus.ply it reads) comes out a closed, consistently oriented surface of
+    Euler characteristic 0, a ball one of characteristic 2 enclosing its
+    centre; every vertex lies in its cube;
+  * the error cases.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import surface as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+F = np.float32
+
+
+def torus():
+    return np.load(os.path.join(GOLDEN, "torus_points.npy"))
+
+
+def torus_run(**kw):
+    args = dict(voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06, isolevel=0.0)
+    args.update(kw)
+    return S.reconstruct(torus(), **args)
+
+
+def test_expf_matches_numpy():
+    xs = np.concatenate([-np.logspace(-8, np.log10(86.0), 4000), np.linspace(-20, 0, 2001)]).astype(np.float32)
+    ours = np.array([S.expf(x) for x in xs], np.float32)
+    ref = np.exp(xs.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(ours - ref) / ulp) <= 2.0
+    assert S.expf(0.0) == 1.0 and S.expf(-100.0) == 0.0 and np.isnan(S.expf(float("nan")))
+
+
+def test_qef_full_rank_is_least_squares():
+    rng = np.random.Generator(np.random.PCG64(5))
+    for k in (3, 4, 6, 12):
+        n = rng.normal(size=(k, 3))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        c = rng.uniform(-0.01, 0.01, size=k)
+        rows = np.concatenate([n, c[:, None]], axis=1).astype(np.float32)
+        x = S.qef(rows)
+        A = rows[:, :3].astype(np.float64)
+        ls = np.linalg.lstsq(A, rows[:, 3].astype(np.float64), rcond=None)[0]
+        s = np.linalg.svd(A, compute_uv=False)
+        if s.min() >= 0.1:
+            assert np.allclose(x, ls, atol=2e-6), k
+
+
+def test_qef_three_planes_meet_in_their_corner():
+    p = np.array([0.003, -0.002, 0.004])
+    normals = np.eye(3)
+    rows = np.array([[*n, n @ p] for n in normals], np.float32)
+    assert np.allclose(S.qef(rows), p, atol=1e-8)
+
+
+def test_qef_rank_deficient_returns_the_minimum_norm_solution():
+    """All normals equal: only the offset along n is fixed; the null space of R
+    is dropped (the reference's SVD would add an arbitrary rotation of it)."""
+    n = np.array([0.6, 0.0, 0.8])
+    c = np.array([0.002, 0.004, 0.003, 0.001])
+    rows = np.array([[*n, ci] for ci in c], np.float32)
+    x = S.qef(rows)
+    assert np.allclose(x, n * c.mean(), atol=1e-7)
+
+
+def test_qef_clamps_small_singular_values():
+    """Two nearly parallel planes: s_min < 0.1, so the component along its
+    singular vector is divided by 0.1, as solve_qef (:70-80) does."""
+    a = np.array([1.0, 0.0, 0.0])
+    b = np.array([np.cos(0.02), np.sin(0.02), 0.0])
+    cz = np.array([0.0, 0.0, 1.0])
+    rows = np.array([[*a, 0.001], [*b, 0.002], [*cz, 0.0005]], np.float32)
+    A = rows[:, :3].astype(np.float64)
+    rhs = rows[:, 3].astype(np.float64)
+    Q, R = np.linalg.qr(A)
+    U, s, Vt = np.linalg.svd(R)
+    assert s.min() < 0.1
+    expect = Vt.T @ np.diag(1.0 / np.maximum(s, 0.1)) @ U.T @ (Q.T @ rhs)
+    assert np.allclose(S.qef(rows), expect, atol=1e-6)
+    assert not np.allclose(expect, np.linalg.solve(A, rhs), atol=1e-4)  # the clamp mattered
+
+
+def _poly6(r, h):
+    r = np.asarray(r, np.float64)
+    return np.where(r <= h, 315.0 / (64.0 * np.pi * h**9) * (h * h - r * r) ** 3, 0.0)
+
+
+def _aniso(dx, G, det):
+    q = np.linalg.norm(dx @ G.T, axis=-1)
+    out = np.zeros_like(q)
+    a = (q > 0) & (q <= 0.5)
+    b = (q > 0.5) & (q <= 1.0)
+    out[a] = 48.0 / np.pi * det * (q[a] ** 3 - q[a] ** 2) + 1.0
+    out[b] = 16.0 / np.pi * det * (1.0 - q[b]) ** 3
+    return out
+
+
+def _vertex_pos(lo, dims, vs, origin=(0.0, 0.0, 0.0)):
+    ix = np.stack(np.meshgrid(*[np.arange(lo[a], lo[a] + dims[a]) for a in range(3)], indexing="ij"), axis=-1)
+    return ix, ix.astype(np.float64) * vs + np.asarray(origin)
+
+
+def test_single_particle_isotropic_field_closed_form():
+    vs, r, h = 0.01, 0.02, 0.03
+    x = np.array([[0.5132, 0.4871, 0.5055]], np.float32)
+    out = S.reconstruct(x, voxel_size=vs, particle_radius=r, smooth_radius=h, anisotropic=False, smooth_iterations=0)
+    ix, p = _vertex_pos(out["lo"], out["dims"], vs)
+    vol = 4.0 / 3.0 * np.pi * r**3
+    c = np.floor(x[0] / vs).astype(int)
+    near = np.all(np.abs(ix - c) <= 1, axis=-1)
+    expect = np.where(near, vol * _poly6(np.linalg.norm(p - x[0].astype(np.float64), axis=-1), h), 0.0)
+    f0 = out["field0"].astype(np.float64)
+    assert np.count_nonzero(f0) == 27
+    # FP32 cancellation in h^2 - r^2 near the support edge: compare on the scale of the peak
+    assert np.max(np.abs(f0 - expect)) <= 1e-5 * np.max(expect)
+    # the written blocks: those of the 27 vertices
+    blocks = {tuple(b) for b in ((ix[near] - np.array(out["lo"])) // 32)}
+    act = {tuple(b) for b in np.argwhere(out["active"])}
+    assert act == blocks
+
+
+def _brute_multiplicity(c, off, R, lo, dims):
+    """Count, per vertex, the (voxel, corner) pairs of the reference's loop
+    (:1142-1155): voxels c - off .. c - off + R - 1 per axis, 8 corners each."""
+    m = np.zeros(dims, np.int64)
+    corners = [(0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 0), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, 1)]
+    for dx in range(R):
+        for dy in range(R):
+            for dz in range(R):
+                n = (c[0] + dx - off, c[1] + dy - off, c[2] + dz - off)
+                for co in corners:
+                    v = [n[a] + co[a] for a in range(3)]
+                    if min(v) < 0:
+                        continue
+                    g = [v[a] - lo[a] for a in range(3)]
+                    m[g[0], g[1], g[2]] += 1
+    return m
+
+
+def test_isolated_particles_anisotropic_field_and_multiplicity():
+    """Particle 0 gets G = 0 (the reference's loop starts at 1); particle 1,
+    alone within 2h, gets G = 2 / h I, x_bar = x, and its box is added with the
+    voxel-corner multiplicities."""
+    vs, r, h = 0.01, 0.02, 0.015
+    x = np.array([[0.2013, 0.2521, 0.2492], [0.4517, 0.2488, 0.2513]], np.float32)
+    out = S.reconstruct(x, voxel_size=vs, particle_radius=r, smooth_radius=h, anisotropic=True, smooth_iterations=0)
+    G, xb = out["G"], out["xbar"]
+    assert np.all(G[0] == 0) and np.all(xb[0] == 0)
+    assert np.allclose(G[1], 2.0 / h * np.eye(3), rtol=1e-6, atol=1e-6)
+    assert np.allclose(xb[1], x[1], atol=1e-7)
+    R = int(np.ceil(F(4.0) * F(h) / F(vs))) + 1
+    off = R // 2
+    lo, dims = out["lo"], out["dims"]
+    ix, p = _vertex_pos(lo, dims, vs)
+    vol = 4.0 / 3.0 * np.pi * r**3
+    expect = np.zeros(dims)
+    for i in range(2):
+        c = np.floor(x[i] / vs).astype(int)
+        near = np.all(np.abs(ix - c) <= 1, axis=-1)
+        expect += np.where(near, vol * _poly6(np.linalg.norm(p - x[i].astype(np.float64), axis=-1), h), 0.0)
+    c = np.floor(x[1] / vs).astype(int)
+    m = _brute_multiplicity(c, off, R, lo, dims)
+    assert set(np.unique(m)) <= {0, 1, 2, 4, 8} and np.count_nonzero(m == 8) > 0 and np.count_nonzero(m == 1) == 8
+    Gd = G[1].astype(np.float64)
+    expect += m * vol * _aniso(xb[1].astype(np.float64) - p, Gd, np.linalg.det(Gd))
+    f0 = out["field0"].astype(np.float64)
+    scale = np.max(np.abs(expect))
+    # FP32: q = |G dx| with |G| ~ 2 / h, and the cancellation in q^3 - q^2
+    assert np.max(np.abs(f0 - expect)) <= 2e-5 * scale
+    assert np.min(f0) < 0  # the reference's "+ 1" placement makes the core negative
+
+
+def test_lattice_particle_covariance_and_G():
+    vs, r, h = 0.01, 0.01, 0.02
+    sp = 0.01
+    g = np.arange(-4, 5) * sp
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + 0.3
+    x = x.astype(np.float32)
+    rng = np.random.Generator(np.random.PCG64(3))
+    x[1:] += rng.uniform(-1e-3, 1e-3, size=(len(x) - 1, 3)).astype(np.float32)
+    out = S.reconstruct(x, voxel_size=vs, particle_radius=r, smooth_radius=h, smooth_iterations=0)
+    i = int(np.argmin(np.linalg.norm(x - 0.3, axis=1)))
+    xi = x[i].astype(np.float64)
+    X = x.astype(np.float64)
+    d = np.linalg.norm(X - xi, axis=1)
+    w = np.where(d < 2 * h, 1.0 - (d / (2 * h)) ** 3, 0.0)
+    xw = (w[:, None] * X).sum(0) / w.sum()
+    D = X - xw
+    C = (w[:, None, None] * D[:, :, None] * D[:, None, :]).sum(0) / w.sum()
+    ev, V = np.linalg.eigh(C)
+    ev, V = ev[::-1], V[:, ::-1]
+    assert np.count_nonzero(w > 0) > 25
+    sg = np.array([ev[0], max(ev[1], ev[0] / 4), max(ev[2], ev[0] / 4)]) * 1400.0
+    G = V @ np.diag(1.0 / sg) @ V.T / h
+    assert np.allclose(out["G"][i], G, rtol=1e-3, atol=1e-3 * np.abs(G).max())
+    assert np.allclose(out["xbar"][i], 0.1 * xi + 0.9 * xw, atol=1e-6)
+
+
+def test_smoothing_sd_and_one_bilateral_pass():
+    x = torus()
+    out = S.reconstruct(x, voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06, smooth_iterations=1)
+    f0 = out["field0"].astype(np.float64)
+    act = np.repeat(np.repeat(np.repeat(out["active"], 32, 0), 32, 1), 32, 2).astype(bool)
+    neg = f0[act & (f0 < 0)]
+    sd = np.sqrt(np.mean((neg - neg.mean()) ** 2))
+    assert abs(out["sd"] - sd) <= 1e-6 * sd
+    # one pass of smooth_sdf_impl (:1459-1484) in FP64 over the written blocks
+    sdf = out["sd"]
+    pad = np.pad(np.where(act, f0, 0.0), 1)
+    num = np.zeros_like(f0)
+    den = np.zeros_like(f0)
+    lo = out["lo"]
+    ok_lo = [np.arange(lo[a], lo[a] + f0.shape[a]) for a in range(3)]
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                nb = pad[1 + dx:1 + dx + f0.shape[0], 1 + dy:1 + dy + f0.shape[1], 1 + dz:1 + dz + f0.shape[2]]
+                valid = ((ok_lo[0] + dx >= 0)[:, None, None] & (ok_lo[1] + dy >= 0)[None, :, None]
+                         & (ok_lo[2] + dz >= 0)[None, None, :])
+                gw = np.exp(-(f0 - nb) ** 2 / (2 * sdf * sdf)) / (2 * np.pi * sdf * sdf) ** 1.5
+                gw = np.where(valid, gw, 0.0)
+                num += gw * nb
+                den += gw
+    expect = np.where(act, num / den, 0.0)
+    got = out["field"].astype(np.float64)
+    assert np.max(np.abs(got - expect)) <= 1e-5 * np.max(np.abs(expect))
+
+
+def _edges(faces):
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    return e
+
+
+def check_closed_oriented(v, faces):
+    """Every undirected edge in exactly 2 triangles, every directed edge once
+    (consistent orientation); returns the Euler characteristic of the used part."""
+    e = _edges(faces)
+    und = np.sort(e, axis=1)
+    _, cnt = np.unique(und, axis=0, return_counts=True)
+    assert np.all(cnt == 2)
+    _, dcnt = np.unique(e, axis=0, return_counts=True)
+    assert np.all(dcnt == 1)
+    used = np.unique(faces)
+    return len(used) - len(cnt) + len(faces)
+
+
+def _signed_volume(v, f):
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    return np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0
+
+
+def test_torus_example_is_a_closed_torus():
+    out = torus_run(normals=True)
+    v = out["vertices"].astype(np.float64)
+    f = out["faces"]
+    assert len(v) > 1000 and len(f) == 2 * len(v)
+    assert check_closed_oriented(v, f) == 0
+    # every vertex lies in its own cube (the QEF result if it stays there, else the mean)
+    n = out["normals"]
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-5)
+    assert abs(_signed_volume(v, f)) > 1e-5
+    in_own_cube_in_grid_order(out["vertices"], 0.01)
+    assert np.allclose(v.mean(axis=0), torus().mean(axis=0), atol=2e-3)
+
+
+def ball(sp=0.02, radius=5.5):
+    g = np.arange(-7, 8) * sp
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    return (x[np.linalg.norm(x, axis=1) <= radius * sp] + 0.5).astype(np.float32)
+
+
+def test_ball_is_a_closed_sphere_around_its_centre():
+    """A lattice ball at the torus example's ratios (spacing 2 voxels, smooth
+    radius 6 voxels) gives a sphere.  Dual contouring can give non-manifold
+    edges where cubes are ambiguous (with the smooth radius at one spacing it
+    does); these settings give none."""
+    out = S.reconstruct(ball(), voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06)
+    v, f = out["vertices"].astype(np.float64), out["faces"]
+    assert check_closed_oriented(v, f) == 2
+    r = np.linalg.norm(v - 0.5, axis=1)
+    assert r.max() < 0.12 and r.min() > 0.08
+    in_own_cube_in_grid_order(out["vertices"], 0.01)
+
+
+def in_own_cube_in_grid_order(v, vs):
+    """Each vertex lies in its own cube (the QEF result when it stays there,
+    else the crossings' mean, which is inside too) and vertices are numbered in
+    grid order of their cubes: the cubes' keys increase strictly."""
+    c = np.floor(v.astype(np.float32) / np.float32(vs)).astype(np.int64)
+    key = (c[:, 0] * 4096 + c[:, 1]) * 4096 + c[:, 2]
+    assert np.all(np.diff(key) > 0)
+
+
+def test_errors():
+    with pytest.raises(S.SurfaceError) as e:
+        S.reconstruct(np.array([[-0.02, 0.1, 0.1]], np.float32))
+    assert e.value.code == S.E_OUT_OF_GRID
+    with pytest.raises(S.SurfaceError) as e:
+        S.reconstruct(np.full((65, 3), 0.105, np.float32))
+    assert e.value.code == S.E_VOXEL_OVERFLOW
+    with pytest.raises(S.SurfaceError) as e:
+        S.reconstruct(np.array([[np.nan, 0.1, 0.1]], np.float32))
+    assert e.value.code == S.E_OUT_OF_GRID
+    # 64 particles in one voxel are allowed
+    S.reconstruct(np.full((64, 3), 0.105, np.float32) + np.linspace(0, 1e-3, 64, dtype=np.float32)[:, None],
+                  anisotropic=False)
+
+
+def test_ply_mesh_export_round_trip(tmp_path):
+    """nonnewton_amd.ply.export_ply_mesh (export_mesh, :1802-1838) writes what
+    the PLY reader reads back; the face records parse as 3 + three int32."""
+    from nonnewton_amd import ply
+
+    out = torus_run(normals=True)
+    path = str(tmp_path / "m.ply")
+    ply.export_ply_mesh(out["vertices"], out["faces"], path, normals=out["normals"])
+    back = ply.read_vertices(path)
+    assert np.array_equal(np.stack([back["x"], back["y"], back["z"]], 1), out["vertices"])
+    assert np.array_equal(np.stack([back["nx"], back["ny"], back["nz"]], 1), out["normals"])
+    raw = open(path, "rb").read()
+    body = raw[raw.index(b"end_header\n") + len(b"end_header\n"):]
+    rec = np.frombuffer(body[len(out["vertices"]) * 24:], dtype=[("n", "u1"), ("i", "<i4", (3,))])
+    assert np.all(rec["n"] == 3) and np.array_equal(rec["i"], out["faces"])

@@ -295,6 +295,68 @@ def copy_probe(mib):
           flush=True)
 
 
+def surface_block(n, sp=0.02, seed=5):
+    """A jittered lattice block of about n particles (aspect 2:1:2, spacing sp)."""
+    k = max(2, round((n / 4.0) ** (1.0 / 3.0)))
+    dims = (2 * k, k, 2 * k)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = [np.arange(d) * sp for d in dims]
+    x = np.stack(np.meshgrid(*g, indexing="ij"), -1).reshape(-1, 3) + 0.1
+    x += rng.uniform(-0.3 * sp, 0.3 * sp, size=x.shape)
+    return x.astype(np.float32)
+
+
+def run_surface(n, reps, cpu_particles):
+    """Surface reconstruction (SRTool.dual_contouring, SPH/surf_recon/srtool.py:1264-1304)
+    of a jittered fluid block at the reference example's ratios (voxel 0.01,
+    particle radius 0.02, smooth radius 0.06, spacing 0.02): positions resident
+    in device memory, one warm-up run, then `reps` timed runs (each a whole
+    reconstruction, host synchronisations included)."""
+    import ctypes
+
+    from nonnewton_amd.surface import Surface
+
+    x = surface_block(n)
+    kw = dict(voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06, isolevel=0.0)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    d = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(d), ctypes.c_size_t(x.nbytes)) != 0:
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemcpy(d, x.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(x.nbytes), 1) != 0:
+        raise RuntimeError("hipMemcpy failed")
+    s = Surface(**kw)
+    info = s.run_device(d.value, len(x))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        info = s.run_device(d.value, len(x))
+    el = time.perf_counter() - t0
+    s.close()
+    hip.hipFree(d)
+    out = {
+        "metric": "surface reconstruction particles/s (SRTool.dual_contouring)",
+        "value": round(len(x) * reps / el, 1),
+        "unit": "particles/s",
+        "ms_per_reconstruction": round(1e3 * el / reps, 3),
+        "reps": reps,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered block)",
+        "config": {"workload": f"jittered block, {len(x)} particles, spacing 0.02", **kw,
+                   "grid_vertices": int(np.prod(info["dims"])), "written_blocks": info["active_blocks"],
+                   "mesh_vertices": info["num_vertices"], "mesh_triangles": info["num_triangles"]},
+    }
+    if cpu_particles > 0:
+        from oracle import surface as O
+
+        xs = surface_block(cpu_particles)
+        t0 = time.perf_counter()
+        O.reconstruct(xs, **kw)
+        ce = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(len(xs) / ce, 1), "unit": "particles/s", "cores": 1, "kind": "port",
+                               "sample": f"one reconstruction of a {len(xs)}-particle block of the same kind "
+                                         f"({ce:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,7 +376,15 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (calibrates the rocprofv3 FETCH_SIZE and "
                          "WRITE_SIZE counters against a kernel of known traffic) and exit")
+    ap.add_argument("--surface", type=int, default=0, metavar="N",
+                    help="only measure the surface reconstruction of an N-particle block and exit")
+    ap.add_argument("--surface-reps", type=int, default=5)
+    ap.add_argument("--surface-cpu", type=int, default=16000, metavar="N",
+                    help="particles of the oracle's CPU sample for --surface (0 = none)")
     args = ap.parse_args()
+    if args.surface:
+        run_surface(args.surface, args.surface_reps, args.surface_cpu)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -7,9 +7,11 @@
 //   hash        per-voxel particle lists, <= 64 each, sorted by particle index
 //               (update_particles / add_particle, :1193-1196, :820-823)
 //   aniso       x_bar, C_i, G_i per particle (pre_process_data, :1007-1112)
-//   splat       one workgroup per particle adds its poly6 box (generate_fluid_field,
-//               :1239-1251) and its anisotropic box (generate_smooth_fluid_field_impl,
-//               :1136-1157) into the vertex field, marking the 32^3 blocks it writes
+//   field       the blocks the particles' boxes write are marked; then one
+//               workgroup per 8^3 vertex tile sums, per vertex, the poly6 values
+//               (generate_fluid_field, :1239-1251) and the anisotropic values
+//               (generate_smooth_fluid_field_impl, :1136-1157) of the particles
+//               whose boxes reach it
 //   stats       FP64 mean and sd of the negative values (density_field_stats, :1437-1456)
 //   smooth      bilateral 27-point pass(es) (smooth_sdf_impl, :1459-1484)
 //   contour     cubes with a crossing numbered by a scan, their dual-contouring
@@ -22,7 +24,8 @@
 // block for "written" (the reference's sparse leaf blocks: its loops visit
 // exactly those, and an unwritten vertex reads 0).  The field is accumulated in
 // 2^-32 fixed point (int64 atomics), so its value does not depend on the order
-// in which the particles' contributions land, then rounded once to FP32.  The
+// in which the particles' contributions are added (a per-vertex gather here, a
+// per-particle scatter in the reference), then rounded once to FP32.  The
 // reference adds FP32 with atomics in no defined order.  Every other
 // expression is FP32 in the reference's operation order (-ffp-contract=off).
 #include <hip/hip_runtime.h>
@@ -53,6 +56,7 @@ struct K {
   int glo[3], gdim[3], bdim[3];
   long nvox, nvert, nblock;
   const float* pos;  // 3 per particle, caller order
+  int* pvox;         // 3 per particle: its voxel
   int *vcount, *vstart, *vslot, *vlist;
   float *G, *xbar, *detG;
   unsigned char* active;
@@ -69,6 +73,8 @@ struct K {
 // ------------------------------------------------------------------ helpers
 
 __device__ __forceinline__ float norm3f(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+// a value every lane of the wavefront holds (lets the compiler use scalar loads)
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // exp for the bilateral weights (x <= 0 there).  Cody-Waite reduction
 // x = k ln2 + r with ln2 split into a 16-bit high part (k ln2_hi exact) and a
@@ -227,10 +233,6 @@ __global__ void k_fill_i32(int* p, int v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_fill_u64(unsigned long long* p, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0ull;
-}
 __global__ void k_fill_u8(unsigned char* p, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -270,6 +272,9 @@ __global__ void k_vcount(K k) {
   if (i >= k.n) return;
   int v[3];
   voxel_of(k, k.pos + 3 * (long)i, v);
+  k.pvox[3 * (long)i] = v[0];
+  k.pvox[3 * (long)i + 1] = v[1];
+  k.pvox[3 * (long)i + 2] = v[2];
   const int slot = atomicAdd(k.vcount + vkey(k, v), 1);
   if (slot >= MAX_PER_VOXEL) atomicOr(k.err, ERR_VOXEL_OVERFLOW);
   k.vslot[i] = slot;
@@ -310,6 +315,44 @@ __device__ __forceinline__ void vrange(const K& k, int x, int y, int z, int& b, 
   e = k.vstart[q + 1];
 }
 
+// The particles a sweep of pre_process_data visits, in its order (voxel dx,
+// dy, dz nested, z fastest, then the voxel's list), restricted to the voxels
+// that can hold a particle closer than r2: a voxel's particles are contiguous
+// in vlist and the voxels of one (x, y) column are consecutive, so each
+// column is one range.  A skipped particle has w = 0 and adds +0 to every sum
+// (rows and column ends are trimmed with a margin of 1e-3 voxel around each
+// voxel and 1e-4 of r2), so the sums are those of the full loop bit for bit.
+template <typename F>
+__device__ __forceinline__ void for_reach(const K& k, const int* v, float x0, float x1, float x2, float r2, F f) {
+  const int R = k.R, off = k.off;
+  const float reach = r2 * 1.0001f;
+  const float reach2 = reach * reach;
+  const float m = 1e-3f * k.vs;
+  for (int dx = 0; dx < R; dx++) {
+    const int x = v[0] + dx - off;
+    if (x < 0 || x < k.vlo[0] || x >= k.vlo[0] + k.vdim[0]) continue;
+    const float lx = (float)x * k.vs + k.o[0] - m, hx = lx + k.vs + 2.0f * m;
+    const float gx = fmaxf(0.0f, fmaxf(lx - x0, x0 - hx));
+    for (int dy = 0; dy < R; dy++) {
+      const int y = v[1] + dy - off;
+      if (y < 0 || y < k.vlo[1] || y >= k.vlo[1] + k.vdim[1]) continue;
+      const float ly = (float)y * k.vs + k.o[1] - m, hy = ly + k.vs + 2.0f * m;
+      const float gy = fmaxf(0.0f, fmaxf(ly - x1, x1 - hy));
+      const float rest = reach2 - (gx * gx + gy * gy);
+      if (!(rest > 0.0f)) continue;
+      const float zr = sqrtf(rest);
+      int z0 = (int)floorf((x2 - zr - k.o[2]) / k.vs) - 1;
+      int z1 = (int)floorf((x2 + zr - k.o[2]) / k.vs) + 1;
+      z0 = max(max(z0, v[2] - off), max(0, k.vlo[2]));
+      z1 = min(min(z1, v[2] - off + R - 1), k.vlo[2] + k.vdim[2] - 1);
+      if (z0 > z1) continue;
+      const long base = ((long)(x - k.vlo[0]) * k.vdim[1] + (y - k.vlo[1])) * k.vdim[2] - k.vlo[2];
+      const int b = k.vstart[base + z0], e = k.vstart[base + z1 + 1];
+      for (int mm = b; mm < e; mm++) f(k.vlist[mm]);
+    }
+  }
+}
+
 // pre_process_data (:1007-1112) for particle i >= 1 (particle 0 keeps G = 0,
 // x_bar = 0: the reference's loop starts at 1 and reset() zeroes both)
 __global__ void k_aniso(K k) {
@@ -323,41 +366,26 @@ __global__ void k_aniso(K k) {
     k.detG[0] = 0.0f;
     return;
   }
-  const int R = k.R, off = k.off;
   const float* xi = k.pos + 3 * (long)i;
   const float x0 = xi[0], x1 = xi[1], x2 = xi[2];
   int v[3];
   voxel_of(k, xi, v);
   const float r2 = 2.0f * k.sr;
   float tw = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
-  for (int dx = 0; dx < R; dx++) {
-    const int x = v[0] + dx - off;
-    if (x < 0) continue;
-    for (int dy = 0; dy < R; dy++) {
-      const int y = v[1] + dy - off;
-      if (y < 0) continue;
-      for (int dz = 0; dz < R; dz++) {
-        const int z = v[2] + dz - off;
-        if (z < 0) continue;
-        int b, e;
-        vrange(k, x, y, z, b, e);
-        for (int m = b; m < e; m++) {
-          const float* xj = k.pos + 3 * (long)k.vlist[m];
-          const float y0 = xj[0], y1 = xj[1], y2 = xj[2];
-          const float nr = norm3f(x0 - y0, x1 - y1, x2 - y2);
-          float w = 0.0f;
-          if (nr < r2) {
-            const float t = nr / r2;
-            w = 1.0f - t * t * t;
-          }
-          tw += w;
-          w0 += w * y0;
-          w1 += w * y1;
-          w2 += w * y2;
-        }
-      }
+  for_reach(k, v, x0, x1, x2, r2, [&](int j) {
+    const float* xj = k.pos + 3 * (long)j;
+    const float y0 = xj[0], y1 = xj[1], y2 = xj[2];
+    const float nr = norm3f(x0 - y0, x1 - y1, x2 - y2);
+    float w = 0.0f;
+    if (nr < r2) {
+      const float t = nr / r2;
+      w = 1.0f - t * t * t;
     }
-  }
+    tw += w;
+    w0 += w * y0;
+    w1 += w * y1;
+    w2 += w * y2;
+  });
   if (tw != 0.0f) {
     w0 /= tw;
     w1 /= tw;
@@ -368,38 +396,24 @@ __global__ void k_aniso(K k) {
   xb[2] = k.lam1 * x2 + k.lam * w2;
   int nn = 0;
   float C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int dx = 0; dx < R; dx++) {
-    const int x = v[0] + dx - off;
-    if (x < 0) continue;
-    for (int dy = 0; dy < R; dy++) {
-      const int y = v[1] + dy - off;
-      if (y < 0) continue;
-      for (int dz = 0; dz < R; dz++) {
-        const int z = v[2] + dz - off;
-        if (z < 0) continue;
-        int b, e;
-        vrange(k, x, y, z, b, e);
-        for (int m = b; m < e; m++) {
-          const float* xj = k.pos + 3 * (long)k.vlist[m];
-          const float y0 = xj[0], y1 = xj[1], y2 = xj[2];
-          const float nr = norm3f(x0 - y0, x1 - y1, x2 - y2);
-          float w = 0.0f;
-          if (nr < r2) {
-            const float t = nr / r2;
-            w = 1.0f - t * t * t;
-          }
-          if (w > 0.0f) {
-            nn++;
-            const float dp[3] = {y0 - w0, y1 - w1, y2 - w2};
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-              for (int c = 0; c < 3; c++) C[3 * a + c] += w * (dp[a] * dp[c]);
-          }
-        }
-      }
+  for_reach(k, v, x0, x1, x2, r2, [&](int j) {
+    const float* xj = k.pos + 3 * (long)j;
+    const float y0 = xj[0], y1 = xj[1], y2 = xj[2];
+    const float nr = norm3f(x0 - y0, x1 - y1, x2 - y2);
+    float w = 0.0f;
+    if (nr < r2) {
+      const float t = nr / r2;
+      w = 1.0f - t * t * t;
     }
-  }
+    if (w > 0.0f) {
+      nn++;
+      const float dp[3] = {y0 - w0, y1 - w1, y2 - w2};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) C[3 * a + c] += w * (dp[a] * dp[c]);
+    }
+  });
   if (tw != 0.0f)
 #pragma unroll
     for (int a = 0; a < 9; a++) C[a] /= tw;
@@ -451,62 +465,152 @@ __global__ void k_aniso(K k) {
   k.detG[i] = g[0] * (g[4] * g[8] - g[7] * g[5]) - g[3] * (g[1] * g[8] - g[7] * g[2]) + g[6] * (g[1] * g[5] - g[4] * g[2]);
 }
 
-// One workgroup per particle: its 27 poly6 vertices, then its anisotropic
-// vertex box [c - off, c - off + R]^3, each vertex with the multiplicity of
-// the reference's voxel-corner loop; the blocks it touches are marked.
-constexpr int SPLAT_THREADS = 256;
-__global__ __launch_bounds__(SPLAT_THREADS) void k_splat(K k) {
-  const int i = blockIdx.x;
+// The blocks the two field passes write: every vertex >= 0 of the particle's
+// 27-vertex poly6 box and of its anisotropic box [c - off, c - off + R]^3
+// (the reference writes them all, zeros included).  Each box spans at most two
+// blocks per axis (R + 1 <= 32 is checked on the host), so its 8 corners name
+// every block it touches.
+__global__ void k_mark(K k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  const int* v = k.pvox + 3 * (long)i;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = v[a] - 1;
+    hi[a] = v[a] + 1;
+    if (k.aniso) {
+      lo[a] = min(lo[a], v[a] - k.off);
+      hi[a] = max(hi[a], v[a] - k.off + k.R);
+    }
+    lo[a] = max(lo[a], 0);
+  }
+  for (int c = 0; c < 8; c++) {
+    const int x = (c >> 2) ? hi[0] : lo[0], y = ((c >> 1) & 1) ? hi[1] : lo[1], z = (c & 1) ? hi[2] : lo[2];
+    k.active[bidx(k, x - k.glo[0], y - k.glo[1], z - k.glo[2])] = 1;
+  }
+}
+
+// The field at the vertices of one 8^3 tile, one thread per vertex, summed
+// over every particle whose boxes can reach the tile: the particles of voxels
+// c in [T - R + off, T + 7 + off] per axis (T = the tile's first vertex).  The
+// workgroup lists the candidate voxel columns and their particle counts (a
+// scan in LDS), then stages the candidates' data in LDS 512 at a time (each
+// thread finds its candidate's column by bisection), and every thread adds
+// what each staged particle gives its vertex: the poly6 value when
+// |v - c_j| <= 1 per axis, the anisotropic value times the multiplicity of the
+// reference's voxel-corner loop when v is in j's box.  The contributions are
+// 2^-32 fixed-point integers, so the sum is the one any order of the
+// reference's atomics would give exactly.  Tiles of unwritten blocks store 0.
+constexpr int TILE = 8;
+constexpr int GATHER_THREADS = TILE * TILE * TILE;
+constexpr int MAX_COLS = 2048;  // (TILE + R)^2 columns, R + 1 <= 32
+__global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
+  __shared__ int col_off[MAX_COLS + 1];
+  __shared__ int col_begin[MAX_COLS];
+  __shared__ float s_pos[3][GATHER_THREADS], s_xb[3][GATHER_THREADS], s_g[9][GATHER_THREADS], s_det[GATHER_THREADS];
+  __shared__ int s_c[3][GATHER_THREADS];
+  const int tb = blockIdx.x;
+  const int tdy = k.gdim[1] / TILE, tdz = k.gdim[2] / TILE;
+  const int tx = tb / (tdy * tdz), ty = (tb / tdz) % tdy, tz = tb % tdz;
   const int t = threadIdx.x;
-  const float* xi = k.pos + 3 * (long)i;
-  int v[3];
-  voxel_of(k, xi, v);
+  const int gx = tx * TILE + (t >> 6), gy = ty * TILE + ((t >> 3) & 7), gz = tz * TILE + (t & 7);
+  const long q = gidx(k, gx, gy, gz);
+  if (!k.active[bidx(k, tx * TILE, ty * TILE, tz * TILE)]) {  // a tile lies in one block
+    k.acc[q] = 0ull;
+    return;
+  }
+  const int vx = gx + k.glo[0], vy = gy + k.glo[1], vz = gz + k.glo[2];
+  float p[3];
+  vpos(k, vx, vy, vz, p);
   const int R = k.R, off = k.off;
-  // blocks: the box spans at most 2 blocks per axis (R + 1 <= 32 is checked on the host)
-  if (t < 8) {
-    const int ax = t >> 2, ay = (t >> 1) & 1, az = t & 1;
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {
-      lo[a] = v[a] - 1;
-      hi[a] = v[a] + 1;
-      if (k.aniso) {
-        lo[a] = min(lo[a], v[a] - off);
-        hi[a] = max(hi[a], v[a] - off + R);
+  const int T[3] = {tx * TILE + k.glo[0], ty * TILE + k.glo[1], tz * TILE + k.glo[2]};
+  int clo[3], chi[3];
+  for (int a = 0; a < 3; a++) {
+    clo[a] = k.aniso ? T[a] - R + off : T[a] - 1;
+    chi[a] = k.aniso ? T[a] + TILE - 1 + off : T[a] + TILE;
+    clo[a] = max(clo[a], k.vlo[a]);
+    chi[a] = min(chi[a], k.vlo[a] + k.vdim[a] - 1);
+  }
+  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1);
+  const int ncol = clo[2] <= chi[2] ? ncx * ncy : 0;  // <= MAX_COLS (checked on the host)
+  // columns: particle counts, then an inclusive scan (Hillis-Steele over
+  // GATHER_THREADS-wide rounds with a carry)
+  int carry = 0;
+  for (int c0 = 0; c0 < ncol; c0 += GATHER_THREADS) {
+    const int c = c0 + t;
+    int cnt = 0;
+    if (c < ncol) {
+      const int cx = clo[0] + c / ncy, cy = clo[1] + c % ncy;
+      const long base = ((long)(cx - k.vlo[0]) * k.vdim[1] + (cy - k.vlo[1])) * k.vdim[2] - k.vlo[2];
+      const int b = k.vstart[base + clo[2]], e = k.vstart[base + chi[2] + 1];
+      col_begin[c] = b;
+      cnt = e - b;
+    }
+    __shared__ int scan[GATHER_THREADS];
+    scan[t] = cnt;
+    __syncthreads();
+    for (int d = 1; d < GATHER_THREADS; d <<= 1) {
+      const int u = t >= d ? scan[t - d] : 0;
+      __syncthreads();
+      scan[t] += u;
+      __syncthreads();
+    }
+    if (c < ncol) col_off[c + 1] = carry + scan[t];
+    if (c0 == 0 && t == 0) col_off[0] = 0;
+    carry += scan[GATHER_THREADS - 1];
+    __syncthreads();
+  }
+  const int total = ncol > 0 ? carry : 0;
+  unsigned long long acc = 0ull;
+  bool bad = false;
+  for (int s0 = 0; s0 < total; s0 += GATHER_THREADS) {
+    const int idx = s0 + t;
+    if (idx < total) {
+      int lo = 0, hi = ncol - 1;  // the column c with col_off[c] <= idx < col_off[c + 1]
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (col_off[mid] <= idx) lo = mid;
+        else hi = mid - 1;
       }
-      lo[a] = max(lo[a], 0);
-    }
-    const int c[3] = {ax ? hi[0] : lo[0], ay ? hi[1] : lo[1], az ? hi[2] : lo[2]};
-    k.active[bidx(k, c[0] - k.glo[0], c[1] - k.glo[1], c[2] - k.glo[2])] = 1;
-  }
-  if (t < 27) {
-    const int x = v[0] + t / 9 - 1, y = v[1] + (t / 3) % 3 - 1, z = v[2] + t % 3 - 1;
-    if (x >= 0 && y >= 0 && z >= 0) {
-      float p[3];
-      vpos(k, x, y, z, p);
-      const float w = k.vol * poly6(k, xi[0] - p[0], xi[1] - p[1], xi[2] - p[2], k.sr);
-      if (!(fabsf(w) <= MAX_CONTRIB)) atomicOr(k.err, ERR_RANGE);
-      else if (w != 0.0f) atomicAdd(k.acc + gidx(k, x - k.glo[0], y - k.glo[1], z - k.glo[2]), fix_of(w));
-    }
-  }
-  if (!k.aniso) return;
-  const float* G = k.G + 9 * (long)i;
-  float g[9];
+      const int j = k.vlist[col_begin[lo] + (idx - col_off[lo])];
+      s_pos[0][t] = k.pos[3 * (long)j];
+      s_pos[1][t] = k.pos[3 * (long)j + 1];
+      s_pos[2][t] = k.pos[3 * (long)j + 2];
+      s_c[0][t] = k.pvox[3 * (long)j];
+      s_c[1][t] = k.pvox[3 * (long)j + 1];
+      s_c[2][t] = k.pvox[3 * (long)j + 2];
+      if (k.aniso) {
+        s_xb[0][t] = k.xbar[3 * (long)j];
+        s_xb[1][t] = k.xbar[3 * (long)j + 1];
+        s_xb[2][t] = k.xbar[3 * (long)j + 2];
 #pragma unroll
-  for (int a = 0; a < 9; a++) g[a] = G[a];
-  const float det = k.detG[i];
-  const float xb0 = k.xbar[3 * (long)i], xb1 = k.xbar[3 * (long)i + 1], xb2 = k.xbar[3 * (long)i + 2];
-  const int S = R + 1, S2 = S * S, S3 = S2 * S;
-  for (int q = t; q < S3; q += SPLAT_THREADS) {
-    const int x = v[0] - off + q / S2, y = v[1] - off + (q / S) % S, z = v[2] - off + q % S;
-    if (x < 0 || y < 0 || z < 0) continue;
-    const int m = mult_axis(x, v[0], off, R) * mult_axis(y, v[1], off, R) * mult_axis(z, v[2], off, R);
-    float p[3];
-    vpos(k, x, y, z, p);
-    const float w = k.vol * aniso_w(k, g, det, xb0 - p[0], xb1 - p[1], xb2 - p[2]);
-    if (!(fabsf(w) <= MAX_CONTRIB)) atomicOr(k.err, ERR_RANGE);
-    else if (w != 0.0f)
-      atomicAdd(k.acc + gidx(k, x - k.glo[0], y - k.glo[1], z - k.glo[2]), (unsigned long long)m * fix_of(w));
+        for (int a = 0; a < 9; a++) s_g[a][t] = k.G[9 * (long)j + a];
+        s_det[t] = k.detG[j];
+      }
+    }
+    __syncthreads();
+    const int nst = min(GATHER_THREADS, total - s0);
+    for (int r = 0; r < nst; r++) {
+      const int c0 = s_c[0][r], c1 = s_c[1][r], c2 = s_c[2][r];
+      if (abs(vx - c0) <= 1 && abs(vy - c1) <= 1 && abs(vz - c2) <= 1) {
+        const float w = k.vol * poly6(k, s_pos[0][r] - p[0], s_pos[1][r] - p[1], s_pos[2][r] - p[2], k.sr);
+        if (!(fabsf(w) <= MAX_CONTRIB)) bad = true;
+        else acc += fix_of(w);
+      }
+      if (!k.aniso) continue;
+      const int mm = mult_axis(vx, c0, off, R) * mult_axis(vy, c1, off, R) * mult_axis(vz, c2, off, R);
+      if (mm == 0) continue;
+      float g[9];
+#pragma unroll
+      for (int a = 0; a < 9; a++) g[a] = s_g[a][r];
+      const float w = k.vol * aniso_w(k, g, s_det[r], s_xb[0][r] - p[0], s_xb[1][r] - p[1], s_xb[2][r] - p[2]);
+      if (!(fabsf(w) <= MAX_CONTRIB)) bad = true;
+      else acc += (unsigned long long)mm * fix_of(w);
+    }
+    __syncthreads();
   }
+  k.acc[q] = acc;
+  if (bad) atomicOr(k.err, ERR_RANGE);
 }
 
 __global__ void k_convert(K k) {
@@ -958,7 +1062,7 @@ struct Surface {
   K k{};
   hipStream_t st = nullptr;
   DBuf<float> pos, G, xbar, detG, field0, field, tmp, verts, normals;
-  DBuf<int> vcount, vstart, vslot, vlist, flag, cube_id, tcount, tstart, faces, err, bounds, scratch;
+  DBuf<int> vcount, vstart, vslot, vlist, pvox, flag, cube_id, tcount, tstart, faces, err, bounds, scratch;
   DBuf<unsigned char> active;
   DBuf<unsigned long long> acc;
   DBuf<double> stats, part;
@@ -970,7 +1074,7 @@ struct Surface {
 
   ~Surface() {
     for (auto* b : {&pos, &G, &xbar, &detG, &field0, &field, &tmp, &verts, &normals}) b->release();
-    for (auto* b : {&vcount, &vstart, &vslot, &vlist, &flag, &cube_id, &tcount, &tstart, &faces, &err, &bounds, &scratch})
+    for (auto* b : {&vcount, &vstart, &vslot, &vlist, &pvox, &flag, &cube_id, &tcount, &tstart, &faces, &err, &bounds, &scratch})
       b->release();
     active.release();
     acc.release();
@@ -1110,6 +1214,7 @@ static int run(Surface& S, const float* dpos, long n) {
   SR_HIP(S.vstart.ensure(nvox + 1), "nn_surface_run: alloc");
   SR_HIP(S.vslot.ensure(n), "nn_surface_run: alloc");
   SR_HIP(S.vlist.ensure(n), "nn_surface_run: alloc");
+  SR_HIP(S.pvox.ensure(3 * n), "nn_surface_run: alloc");
   SR_HIP(S.G.ensure(9 * n), "nn_surface_run: alloc");
   SR_HIP(S.xbar.ensure(3 * n), "nn_surface_run: alloc");
   SR_HIP(S.detG.ensure(n), "nn_surface_run: alloc");
@@ -1128,6 +1233,7 @@ static int run(Surface& S, const float* dpos, long n) {
   k.vstart = S.vstart.p;
   k.vslot = S.vslot.p;
   k.vlist = S.vlist.p;
+  k.pvox = S.pvox.p;
   k.G = S.G.p;
   k.xbar = S.xbar.p;
   k.detG = S.detG.p;
@@ -1150,8 +1256,8 @@ static int run(Surface& S, const float* dpos, long n) {
   // anisotropic kernel and field
   if (k.aniso) hipLaunchKernelGGL(k_aniso, dim3(grid1(n, 64)), dim3(64), 0, S.st, k);
   hipLaunchKernelGGL(k_fill_u8, dim3(grid1(nblock, 256)), dim3(256), 0, S.st, k.active, nblock);
-  hipLaunchKernelGGL(k_fill_u64, dim3(grid1(nvert, 256)), dim3(256), 0, S.st, k.acc, nvert);
-  hipLaunchKernelGGL(k_splat, dim3((unsigned)n), dim3(SPLAT_THREADS), 0, S.st, k);
+  hipLaunchKernelGGL(k_mark, dim3(grid1(n, 256)), dim3(256), 0, S.st, k);
+  hipLaunchKernelGGL(k_field, dim3((unsigned)(nvert / GATHER_THREADS)), dim3(GATHER_THREADS), 0, S.st, k);
   hipLaunchKernelGGL(k_convert, dim3(grid1(nvert, 256)), dim3(256), 0, S.st, k);
   if ((rc = device_error(S))) return rc;
   // smoothing

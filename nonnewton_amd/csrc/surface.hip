@@ -238,28 +238,47 @@ __global__ void k_fill_u8(unsigned char* p, long n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void k_bounds(K k) {
+// voxel range of the particles: a workgroup min/max in LDS, then one atomic
+// per bound per workgroup (integers: the order does not matter)
+constexpr int BOUNDS_THREADS = 256;
+__global__ __launch_bounds__(BOUNDS_THREADS) void k_bounds(K k) {
+  __shared__ int red[6][BOUNDS_THREADS];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.n) return;
-  const float* p = k.pos + 3 * (long)i;
-  for (int a = 0; a < 3; a++) {
-    const float t = (p[a] - k.o[a]) / k.vs;
-    if (!(t > -1.0f && t < 1.0e8f)) {
-      atomicOr(k.err, ERR_OUT_OF_GRID);
-      return;
+  const int t = threadIdx.x;
+  int v[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, u[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  if (i < k.n) {
+    const float* p = k.pos + 3 * (long)i;
+    bool ok = true;
+    for (int a = 0; a < 3; a++) {
+      const float tt = (p[a] - k.o[a]) / k.vs;
+      if (!(tt > -1.0f && tt < 1.0e8f)) ok = false;
     }
-  }
-  int v[3];
-  voxel_of(k, p, v);
-  for (int a = 0; a < 3; a++) {
-    if (v[a] < 0) {
-      atomicOr(k.err, ERR_OUT_OF_GRID);
-      return;
+    if (ok) {
+      int w[3];
+      voxel_of(k, p, w);
+      for (int a = 0; a < 3; a++)
+        if (w[a] < 0) ok = false;
+      if (ok)
+        for (int a = 0; a < 3; a++) v[a] = u[a] = w[a];
     }
+    if (!ok) atomicOr(k.err, ERR_OUT_OF_GRID);
   }
   for (int a = 0; a < 3; a++) {
-    atomicMin(k.bounds + a, v[a]);
-    atomicMax(k.bounds + 3 + a, v[a]);
+    red[a][t] = v[a];
+    red[3 + a][t] = u[a];
+  }
+  __syncthreads();
+  for (int span = BOUNDS_THREADS / 2; span > 0; span >>= 1) {
+    if (t < span)
+      for (int a = 0; a < 3; a++) {
+        red[a][t] = min(red[a][t], red[a][t + span]);
+        red[3 + a][t] = max(red[3 + a][t], red[3 + a][t + span]);
+      }
+    __syncthreads();
+  }
+  if (t < 3 && red[t][0] != INT32_MAX) {
+    atomicMin(k.bounds + t, red[t][0]);
+    atomicMax(k.bounds + 3 + t, red[3 + t][0]);
   }
 }
 
@@ -1173,7 +1192,7 @@ static int run(Surface& S, const float* dpos, long n) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S.st, k.err, 0, 1L);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S.st, k.bounds, INT32_MAX, 3L);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S.st, k.bounds + 3, INT32_MIN, 3L);
-  hipLaunchKernelGGL(k_bounds, dim3(grid1(n, 256)), dim3(256), 0, S.st, k);
+  hipLaunchKernelGGL(k_bounds, dim3(grid1(n, BOUNDS_THREADS)), dim3(BOUNDS_THREADS), 0, S.st, k);
   int hb[6];
   SR_HIP(hipMemcpyAsync(hb, k.bounds, sizeof(hb), hipMemcpyDeviceToHost, S.st), "nn_surface_run: bounds");
   int rc = device_error(S);

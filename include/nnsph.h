@@ -258,7 +258,7 @@ typedef struct nn_surface_params {
   int32_t anisotropic;       /* use_smooth_fluid_field (default 1): add the anisotropic field */
   int32_t smooth_iterations; /* smooth_sdf(iteration) (:1425), 1 in dual_contouring (:1289); 0..64 */
   int32_t normals;           /* record_normals (:891): per-vertex normals (process_normal, :1384-1410) */
-  int32_t reserved;
+  int32_t velocity;          /* record_velocity (:892): velocity channels (:1160-1190, :1352-1381) */
 } nn_surface_params;
 
 /* SRTool(...) (srtool.py:887-991).  NN_E_UNSUPPORTED when the anisotropic box
@@ -276,6 +276,13 @@ int32_t nn_surface_set_budget(nn_surface *s, int64_t bytes);
 int32_t nn_surface_run(nn_surface *s, const float *pos, int64_t n);
 /* The same on positions already in device memory (n x 3 float, caller order). */
 int32_t nn_surface_run_device(nn_surface *s, const float *device_pos, int64_t n);
+/* dual_contouring(..., velocity) with record_velocity (params.velocity set):
+ * the particles' velocities (n x 3) are splatted with the poly6 weight times
+ * voxel_size^3 (generate_smooth_fluid_field_with_velocity_impl, :1160-1190) and
+ * interpolated at the mesh vertices (process_velocity, :1378-1381). */
+int32_t nn_surface_run_velocity(nn_surface *s, const float *pos, const float *vel, int64_t n);
+/* mesh_velocity of the last nn_surface_run_velocity (export_mesh's vx, vy, vz). */
+int32_t nn_surface_mesh_velocity(nn_surface *s, float *out);
 /* num_vertices / num_indices of the last run and its grid (srtool.py:917-918):
  * out[0..2] first grid vertex, out[3..5] grid dims (vertices), out[6] vertices,
  * out[7] triangles, out[8] written 32^3 blocks, out[9..11] first voxel of the

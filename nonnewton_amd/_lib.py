@@ -100,7 +100,7 @@ class NnSurfaceParams(ctypes.Structure):
         ("anisotropic", ctypes.c_int32),
         ("smooth_iterations", ctypes.c_int32),
         ("normals", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("velocity", ctypes.c_int32),
     ]
 
 
@@ -182,6 +182,8 @@ def load():
         "nn_surface_mesh": (i32, [vp, P(ctypes.c_float), P(i32), P(ctypes.c_float)]),
         "nn_surface_field": (i32, [vp, i32, P(ctypes.c_float), sz]),
         "nn_surface_aniso": (i32, [vp, P(ctypes.c_float), P(ctypes.c_float), i64]),
+        "nn_surface_run_velocity": (i32, [vp, P(ctypes.c_float), P(ctypes.c_float), i64]),
+        "nn_surface_mesh_velocity": (i32, [vp, P(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

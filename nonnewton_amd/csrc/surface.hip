@@ -56,6 +56,10 @@ struct K {
   int glo[3], gdim[3], bdim[3];
   long nvox, nvert, nblock;
   const float* pos;  // 3 per particle, caller order
+  const float* vel;  // 3 per particle when the velocity channels are on, else null
+  unsigned long long* vacc;  // 3 per vertex (velocity channels)
+  float *vfield, *mvel;      // 3 per vertex, 3 per mesh vertex
+  float cvs3;                // voxel_size^3 (contribution_coeff, :1165)
   int* pvox;         // 3 per particle: its voxel
   int *vcount, *vstart, *vslot, *vlist;
   float *G, *xbar, *detG;
@@ -522,12 +526,13 @@ __global__ void k_mark(K k) {
 // reference's atomics would give exactly.  Tiles of unwritten blocks store 0.
 constexpr int TILE = 8;
 constexpr int GATHER_THREADS = TILE * TILE * TILE;
-constexpr int MAX_COLS = 2048;  // (TILE + R)^2 columns, R + 1 <= 32
+constexpr int MAX_COLS = 1536;  // (TILE + R)^2 columns, R + 1 <= 32
 __global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
   __shared__ int col_off[MAX_COLS + 1];
   __shared__ int col_begin[MAX_COLS];
   __shared__ float s_pos[3][GATHER_THREADS], s_xb[3][GATHER_THREADS], s_g[9][GATHER_THREADS], s_det[GATHER_THREADS];
   __shared__ int s_c[3][GATHER_THREADS];
+  __shared__ float s_vel[3][GATHER_THREADS];
   const int tb = blockIdx.x;
   const int tdy = k.gdim[1] / TILE, tdz = k.gdim[2] / TILE;
   const int tx = tb / (tdy * tdz), ty = (tb / tdz) % tdy, tz = tb % tdz;
@@ -536,6 +541,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
   const long q = gidx(k, gx, gy, gz);
   if (!k.active[bidx(k, tx * TILE, ty * TILE, tz * TILE)]) {  // a tile lies in one block
     k.acc[q] = 0ull;
+    if (k.vel)
+      for (int c = 0; c < 3; c++) k.vacc[3 * q + c] = 0ull;
     return;
   }
   const int vx = gx + k.glo[0], vy = gy + k.glo[1], vz = gz + k.glo[2];
@@ -543,6 +550,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
   vpos(k, vx, vy, vz, p);
   const int R = k.R, off = k.off;
   const int T[3] = {tx * TILE + k.glo[0], ty * TILE + k.glo[1], tz * TILE + k.glo[2]};
+  unsigned long long va[3] = {0ull, 0ull, 0ull};
   int clo[3], chi[3];
   for (int a = 0; a < 3; a++) {
     clo[a] = k.aniso ? T[a] - R + off : T[a] - 1;
@@ -605,6 +613,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
 #pragma unroll
         for (int a = 0; a < 9; a++) s_g[a][t] = k.G[9 * (long)j + a];
         s_det[t] = k.detG[j];
+        if (k.vel)
+          for (int a = 0; a < 3; a++) s_vel[a][t] = k.vel[3 * (long)j + a];
       }
     }
     __syncthreads();
@@ -625,10 +635,21 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
       const float w = k.vol * aniso_w(k, g, s_det[r], s_xb[0][r] - p[0], s_xb[1][r] - p[1], s_xb[2][r] - p[2]);
       if (!(fabsf(w) <= MAX_CONTRIB)) bad = true;
       else acc += (unsigned long long)mm * fix_of(w);
+      if (k.vel) {  // generate_smooth_fluid_field_with_velocity_impl (:1160-1190)
+        const float f = poly6(k, s_pos[0][r] - p[0], s_pos[1][r] - p[1], s_pos[2][r] - p[2], k.sr) * k.cvs3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const float u = s_vel[c][r] * f;
+          if (!(fabsf(u) <= MAX_CONTRIB)) bad = true;
+          else va[c] += (unsigned long long)mm * fix_of(u);
+        }
+      }
     }
     __syncthreads();
   }
   k.acc[q] = acc;
+  if (k.vel)
+    for (int c = 0; c < 3; c++) k.vacc[3 * q + c] = va[c];
   if (bad) atomicOr(k.err, ERR_RANGE);
 }
 
@@ -640,6 +661,10 @@ __global__ void k_convert(K k) {
   if (k.active[bidx(k, gx, gy, gz)]) v = (float)((double)(long long)k.acc[q] * FIX_INV);
   k.field0[q] = v;
   k.field[q] = v;
+  if (k.vel) {
+    const bool act = k.active[bidx(k, gx, gy, gz)];
+    for (int c = 0; c < 3; c++) k.vfield[3 * q + c] = act ? (float)((double)(long long)k.vacc[3 * q + c] * FIX_INV) : 0.0f;
+  }
 }
 
 // Blocked FP64 statistic of the negative values, one workgroup per block in
@@ -1046,11 +1071,49 @@ __global__ void k_faces(K k) {
   }
 }
 
-__global__ void k_normals(K k, long nv) {
+// get_velocity_at (:1352-1359): 0 outside written blocks
+__device__ __forceinline__ void vel_at(const K& k, int x, int y, int z, float* out) {
+  const int gx = x - k.glo[0], gy = y - k.glo[1], gz = z - k.glo[2];
+  if (gx < 0 || gy < 0 || gz < 0 || gx >= k.gdim[0] || gy >= k.gdim[1] || gz >= k.gdim[2] ||
+      !k.active[bidx(k, gx, gy, gz)]) {
+    out[0] = out[1] = out[2] = 0.0f;
+    return;
+  }
+  const long q = gidx(k, gx, gy, gz);
+  for (int c = 0; c < 3; c++) out[c] = k.vfield[3 * q + c];
+}
+
+// per mesh vertex: the normal (process_normal, :1384-1410) and the velocity
+// (process_velocity_at with trilinear_interpolate, :1362-1375, :1324-1333)
+__global__ void k_vertex_attrs(K k, long nv, int want_normals, int want_vel) {
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
   float p[3] = {k.verts[3 * v], k.verts[3 * v + 1], k.verts[3 * v + 2]};
-  normal_at(k, p, k.normals + 3 * v);
+  if (want_normals) normal_at(k, p, k.normals + 3 * v);
+  if (!want_vel) return;
+  float* out = k.mvel + 3 * v;
+  if (!k.vel) {
+    out[0] = out[1] = out[2] = 0.0f;
+    return;
+  }
+  int c3[3];
+  voxel_of(k, p, c3);
+  float base[3];
+  vpos(k, c3[0], c3[1], c3[2], base);
+  const float xd = (p[0] - base[0]) / k.vs, yd = (p[1] - base[1]) / k.vs, zd = (p[2] - base[2]) / k.vs;
+  float c[8][3];
+#pragma unroll
+  for (int j = 0; j < 8; j++) vel_at(k, c3[0] + (j >> 2), c3[1] + ((j >> 1) & 1), c3[2] + (j & 1), c[j]);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float c00 = c[0][a] * (1.0f - xd) + c[4][a] * xd;
+    const float c01 = c[1][a] * (1.0f - xd) + c[5][a] * xd;
+    const float c10 = c[2][a] * (1.0f - xd) + c[6][a] * xd;
+    const float c11 = c[3][a] * (1.0f - xd) + c[7][a] * xd;
+    const float c0 = c00 * (1.0f - yd) + c10 * yd;
+    const float c1 = c01 * (1.0f - yd) + c11 * yd;
+    out[a] = c0 * (1.0f - zd) + c1 * zd;
+  }
 }
 
 // ------------------------------------------------------------------ host
@@ -1080,10 +1143,10 @@ struct Surface {
   nn_surface_params prm{};
   K k{};
   hipStream_t st = nullptr;
-  DBuf<float> pos, G, xbar, detG, field0, field, tmp, verts, normals;
+  DBuf<float> pos, vel, G, xbar, detG, field0, field, tmp, verts, normals, vfield, mvel;
   DBuf<int> vcount, vstart, vslot, vlist, pvox, flag, cube_id, tcount, tstart, faces, err, bounds, scratch;
   DBuf<unsigned char> active;
-  DBuf<unsigned long long> acc;
+  DBuf<unsigned long long> acc, vacc;
   DBuf<double> stats, part;
   DBuf<long long> pcount;
   long nverts = 0, ntris = 0, nactive = 0;
@@ -1092,11 +1155,12 @@ struct Surface {
   int64_t budget = 0;  // device bytes a run may use (0 = no limit)
 
   ~Surface() {
-    for (auto* b : {&pos, &G, &xbar, &detG, &field0, &field, &tmp, &verts, &normals}) b->release();
+    for (auto* b : {&pos, &vel, &G, &xbar, &detG, &field0, &field, &tmp, &verts, &normals, &vfield, &mvel}) b->release();
     for (auto* b : {&vcount, &vstart, &vslot, &vlist, &pvox, &flag, &cube_id, &tcount, &tstart, &faces, &err, &bounds, &scratch})
       b->release();
     active.release();
     acc.release();
+    vacc.release();
     stats.release();
     part.release();
     pcount.release();
@@ -1178,11 +1242,12 @@ static int device_error(Surface& S) {
   return NN_OK;
 }
 
-static int run(Surface& S, const float* dpos, long n) {
+static int run(Surface& S, const float* dpos, const float* dvel, long n) {
   K& k = S.k;
   S.have = false;
   k.n = (int)n;
   k.pos = dpos;
+  k.vel = (S.prm.velocity && k.aniso) ? dvel : nullptr;  // the reference samples velocity in the smooth field only
   SR_HIP(S.err.ensure(1), "nn_surface_run: alloc");
   SR_HIP(S.bounds.ensure(6), "nn_surface_run: alloc");
   SR_HIP(S.stats.ensure(4), "nn_surface_run: alloc");
@@ -1223,7 +1288,7 @@ static int run(Surface& S, const float* dpos, long n) {
   k.nvox = nvox;
   k.nvert = nvert;
   k.nblock = nblock;
-  const double need = (double)nvox * 8 + (double)nvert * 36 + (double)n * 64 + (double)nblock * 20;
+  const double need = (double)nvox * 8 + (double)nvert * (k.vel ? 72 : 36) + (double)n * 64 + (double)nblock * 20;
   if (nvert > (1L << 31) - 2 || nvox > (1L << 31) - 2 || (S.budget > 0 && need > (double)S.budget)) {
     set_error("nn_surface_run: the vertex grid over the particles' extent exceeds the device budget (" +
               std::to_string(nvert) + " vertices)");
@@ -1248,6 +1313,12 @@ static int run(Surface& S, const float* dpos, long n) {
   SR_HIP(S.tstart.ensure(nvert + 1), "nn_surface_run: alloc");
   SR_HIP(S.part.ensure(nblock), "nn_surface_run: alloc");
   SR_HIP(S.pcount.ensure(nblock), "nn_surface_run: alloc");
+  if (k.vel) {
+    SR_HIP(S.vacc.ensure(3 * nvert), "nn_surface_run: alloc");
+    SR_HIP(S.vfield.ensure(3 * nvert), "nn_surface_run: alloc");
+    k.vacc = S.vacc.p;
+    k.vfield = S.vfield.p;
+  }
   k.vcount = S.vcount.p;
   k.vstart = S.vstart.p;
   k.vslot = S.vslot.p;
@@ -1305,10 +1376,17 @@ static int run(Surface& S, const float* dpos, long n) {
   SR_HIP(S.faces.ensure(3 * (size_t)nt), "nn_surface_run: alloc");
   k.faces = S.faces.p;
   hipLaunchKernelGGL(k_faces, dim3(grid1(nvert, 128)), dim3(128), 0, S.st, k);
-  if (S.prm.normals && nv > 0) {
-    SR_HIP(S.normals.ensure(3 * (size_t)nv), "nn_surface_run: alloc");
-    k.normals = S.normals.p;
-    hipLaunchKernelGGL(k_normals, dim3(grid1(nv, 128)), dim3(128), 0, S.st, k, (long)nv);
+  const int want_n = S.prm.normals ? 1 : 0, want_v = S.prm.velocity ? 1 : 0;
+  if ((want_n || want_v) && nv > 0) {
+    if (want_n) {
+      SR_HIP(S.normals.ensure(3 * (size_t)nv), "nn_surface_run: alloc");
+      k.normals = S.normals.p;
+    }
+    if (want_v) {
+      SR_HIP(S.mvel.ensure(3 * (size_t)nv), "nn_surface_run: alloc");
+      k.mvel = S.mvel.p;
+    }
+    hipLaunchKernelGGL(k_vertex_attrs, dim3(grid1(nv, 128)), dim3(128), 0, S.st, k, (long)nv, want_n, want_v);
   }
   double st4[4];
   SR_HIP(hipMemcpyAsync(st4, k.stats, sizeof(st4), hipMemcpyDeviceToHost, S.st), "nn_surface_run: stats");
@@ -1369,6 +1447,7 @@ int32_t nn_surface_create(const nn_surface_params* p, nn_surface** out) {
   k.ks = 1400.0f;
   k.kn = 0.5f;
   k.c2pi = (float)(2 * M_PI);
+  k.cvs3 = (float)(p->voxel_size * p->voxel_size * p->voxel_size);
   k.aniso = p->anisotropic ? 1 : 0;
   k.R = (int)ceilf(4.0f * k.sr / k.vs) + 1;  // smooth_range, :1137
   k.off = k.R / 2;
@@ -1407,7 +1486,7 @@ int32_t nn_surface_run_device(nn_surface* s, const float* dpos, int64_t n) {
     nn::set_error("nn_surface_run_device: bad arguments (n must be in [1, 2^30])");
     return NN_E_ARG;
   }
-  return nn::sr::run(*S, dpos, (long)n);
+  return nn::sr::run(*S, dpos, nullptr, (long)n);
 }
 
 int32_t nn_surface_run(nn_surface* s, const float* pos, int64_t n) {
@@ -1420,7 +1499,43 @@ int32_t nn_surface_run(nn_surface* s, const float* pos, int64_t n) {
   if (e != hipSuccess) return nn::hip_fail(e, "nn_surface_run: alloc");
   e = hipMemcpyAsync(S->pos.p, pos, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, S->st);
   if (e != hipSuccess) return nn::hip_fail(e, "nn_surface_run: upload");
-  return nn::sr::run(*S, S->pos.p, (long)n);
+  return nn::sr::run(*S, S->pos.p, nullptr, (long)n);
+}
+
+int32_t nn_surface_run_velocity(nn_surface* s, const float* pos, const float* vel, int64_t n) {
+  Surface* S = s ? &s->impl : nullptr;
+  if (!S || !pos || !vel || n <= 0 || n > (1L << 30)) {
+    nn::set_error("nn_surface_run_velocity: bad arguments (n must be in [1, 2^30])");
+    return NN_E_ARG;
+  }
+  if (!S->prm.velocity) {
+    nn::set_error("nn_surface_run_velocity: the velocity channels were not requested (nn_surface_params.velocity)");
+    return NN_E_STATE;
+  }
+  hipError_t e = S->pos.ensure(3 * (size_t)n);
+  if (e == hipSuccess) e = S->vel.ensure(3 * (size_t)n);
+  if (e != hipSuccess) return nn::hip_fail(e, "nn_surface_run_velocity: alloc");
+  e = hipMemcpyAsync(S->pos.p, pos, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, S->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(S->vel.p, vel, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, S->st);
+  if (e != hipSuccess) return nn::hip_fail(e, "nn_surface_run_velocity: upload");
+  return nn::sr::run(*S, S->pos.p, S->vel.p, (long)n);
+}
+
+int32_t nn_surface_mesh_velocity(nn_surface* s, float* out) {
+  Surface* S = s ? &s->impl : nullptr;
+  if (!S || (!out && S->nverts)) {
+    nn::set_error("nn_surface_mesh_velocity: bad arguments");
+    return NN_E_ARG;
+  }
+  if (!S->have || !S->prm.velocity) {
+    nn::set_error("nn_surface_mesh_velocity: no completed run with the velocity channels");
+    return NN_E_STATE;
+  }
+  if (S->nverts) {
+    hipError_t e = hipMemcpy(out, S->k.mvel, sizeof(float) * 3 * S->nverts, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return nn::hip_fail(e, "nn_surface_mesh_velocity");
+  }
+  return NN_OK;
 }
 
 int32_t nn_surface_info(nn_surface* s, int64_t* out, double* sd) {

@@ -105,8 +105,8 @@ def export_ply_particles(particles: np.ndarray, filename: str) -> None:
         f.write(p.tobytes())
 
 
-def export_ply_mesh(vertices: np.ndarray, faces: np.ndarray, filename: str, normals=None) -> None:
-    """Binary little-endian PLY mesh: vertex x, y, z (and nx, ny, nz) as f4, faces
+def export_ply_mesh(vertices: np.ndarray, faces: np.ndarray, filename: str, normals=None, velocity=None) -> None:
+    """Binary little-endian PLY mesh: vertex x, y, z (and nx, ny, nz; vx, vy, vz) as f4, faces
     as `list uchar int vertex_indices` (SRTool.export_mesh, SPH/surf_recon/srtool.py:1802-1838)."""
     v = np.ascontiguousarray(vertices, dtype="<f4").reshape(-1, 3)
     f = np.ascontiguousarray(faces, dtype="<i4").reshape(-1, 3)
@@ -117,6 +117,12 @@ def export_ply_mesh(vertices: np.ndarray, faces: np.ndarray, filename: str, norm
             raise ValueError("normals must have one entry per vertex")
         v = np.concatenate([v, n], axis=1)
         props += "property float nx\nproperty float ny\nproperty float nz\n"
+    if velocity is not None:
+        u = np.ascontiguousarray(velocity, dtype="<f4").reshape(-1, 3)
+        if u.shape[0] != v.shape[0]:
+            raise ValueError("velocity must have one entry per vertex")
+        v = np.concatenate([v, u], axis=1)
+        props += "property float vx\nproperty float vy\nproperty float vz\n"
     header = (
         f"ply\nformat binary_little_endian 1.0\nelement vertex {v.shape[0]}\n{props}"
         f"element face {f.shape[0]}\nproperty list uchar int vertex_indices\nend_header\n"

@@ -40,7 +40,7 @@ class Surface:
     """One nn_surface object: fixed parameters, reusable device buffers."""
 
     def __init__(self, origin=(0.0, 0.0, 0.0), voxel_size=0.01, particle_radius=0.01, smooth_radius=0.04,
-                 isolevel=0.0, anisotropic=True, smooth_iterations=1, normals=False):
+                 isolevel=0.0, anisotropic=True, smooth_iterations=1, normals=False, velocity=False):
         lib = L.load()
         p = L.NnSurfaceParams()
         p.origin[:] = [float(v) for v in origin]
@@ -51,6 +51,7 @@ class Surface:
         p.anisotropic = int(bool(anisotropic))
         p.smooth_iterations = int(smooth_iterations)
         p.normals = int(bool(normals))
+        p.velocity = int(bool(velocity))
         self.params = p
         self._h = ctypes.c_void_p()
         L.check(lib.nn_surface_create(ctypes.byref(p), ctypes.byref(self._h)), "nn_surface_create")
@@ -70,11 +71,18 @@ class Surface:
     def set_budget(self, nbytes: int):
         L.check(L.load().nn_surface_set_budget(self._h, int(nbytes)), "nn_surface_set_budget")
 
-    def run(self, pos) -> dict:
-        """Reconstruct from (n, 3) host positions."""
+    def run(self, pos, vel=None) -> dict:
+        """Reconstruct from (n, 3) host positions (and velocities, for an object
+        made with velocity=True)."""
         x = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).reshape(-1, 3))
         self.n = len(x)
-        L.check(L.load().nn_surface_run(self._h, L.fptr(x), len(x)), "nn_surface_run")
+        if vel is None:
+            L.check(L.load().nn_surface_run(self._h, L.fptr(x), len(x)), "nn_surface_run")
+        else:
+            u = np.ascontiguousarray(np.asarray(vel, dtype=np.float32).reshape(-1, 3))
+            if u.shape != x.shape:
+                raise ValueError("velocities must have one (vx, vy, vz) per particle")
+            L.check(L.load().nn_surface_run_velocity(self._h, L.fptr(x), L.fptr(u), len(x)), "nn_surface_run_velocity")
         return self.info()
 
     def run_device(self, ptr: int, n: int) -> dict:
@@ -102,6 +110,12 @@ class Surface:
                 "nn_surface_mesh")
         return v, f, n
 
+    def mesh_velocity(self) -> np.ndarray:
+        """(V, 3) velocities at the mesh vertices of the last run_velocity."""
+        out = np.empty((self.info()["num_vertices"], 3), np.float32)
+        L.check(L.load().nn_surface_mesh_velocity(self._h, L.fptr(out)), "nn_surface_mesh_velocity")
+        return out
+
     def field(self, which: int = 1) -> np.ndarray:
         """The vertex field on the grid (dims from info()): 0 before smoothing, 1 after."""
         dims = self.info()["dims"]
@@ -123,13 +137,11 @@ class SRTool:
     def __init__(self, bounding_box_extent, voxel_size=0.01, particle_radius=0.01, record_normals=False,
                  record_velocity=False, max_num_vertices=8000000, max_num_indices=100000000,
                  max_num_particles=9000000, hash_level_dimensions=None):
-        if record_velocity:
-            raise NotImplementedError("record_velocity: the velocity channels of the mesh are not built")
         self.bounding_box_extent = [list(map(float, bounding_box_extent[0])), list(map(float, bounding_box_extent[1]))]
         self.voxel_size = float(voxel_size)
         self.radius = float(particle_radius)
         self.record_normals = bool(record_normals)
-        self.record_velocity = False
+        self.record_velocity = bool(record_velocity)
         self.max_num_vertices = int(max_num_vertices)
         self.max_num_indices = int(max_num_indices)
         self.max_num_particles = int(max_num_particles)
@@ -143,6 +155,7 @@ class SRTool:
         self.mesh_vertex = np.zeros((0, 3), np.float32)
         self.mesh_index = np.zeros(0, np.int32)
         self.mesh_normal = np.zeros((0, 3), np.float32) if self.record_normals else None
+        self.mesh_velocity = np.zeros((0, 3), np.float32) if self.record_velocity else None
 
     def _surface(self, isolevel, smooth_radius, anisotropic) -> Surface:
         key = (float(isolevel), float(smooth_radius), bool(anisotropic))
@@ -151,7 +164,8 @@ class SRTool:
                 self._surf.close()
             self._surf = Surface(origin=self.bounding_box_extent[0], voxel_size=self.voxel_size,
                                  particle_radius=self.radius, smooth_radius=smooth_radius, isolevel=isolevel,
-                                 anisotropic=anisotropic, smooth_iterations=1, normals=self.record_normals)
+                                 anisotropic=anisotropic, smooth_iterations=1, normals=self.record_normals,
+                                 velocity=self.record_velocity)
             self._key = key
         return self._surf
 
@@ -165,7 +179,13 @@ class SRTool:
         self.reset()
         s = self._surface(isolevel, smooth_radius, use_smooth_fluid_field)
         p = pos.to_numpy() if hasattr(pos, "to_numpy") else pos
-        s.run(p[:n])
+        if self.record_velocity:
+            if velocity is None:
+                raise ValueError("SRTool(record_velocity=True) needs the particles' velocity")
+            u = velocity.to_numpy() if hasattr(velocity, "to_numpy") else velocity
+            s.run(p[:n], u[:n])
+        else:
+            s.run(p[:n])
         v, f, nrm = s.mesh()
         if len(v) > self.max_num_vertices or 3 * len(f) > self.max_num_indices:
             raise ValueError(f"mesh of {len(v)} vertices / {3 * len(f)} indices exceeds max_num_vertices "
@@ -173,15 +193,18 @@ class SRTool:
         self.mesh_vertex = v
         self.mesh_index = f.reshape(-1)
         self.mesh_normal = nrm
+        if self.record_velocity:
+            self.mesh_velocity = s.mesh_velocity()
         self.num_vertices = _Scalar(len(v))
         self.num_indices = _Scalar(3 * len(f))
 
     def export_mesh(self, file_path: str, output_normal=False, output_velocity=False):
         """srtool.py:1802-1838: binary PLY of the last mesh."""
         assert file_path[-3:] == "ply", "Output file must be in ply format!"
-        if output_velocity:
-            raise NotImplementedError("output_velocity: the velocity channels of the mesh are not built")
+        if output_velocity and self.mesh_velocity is None:
+            raise ValueError("output_velocity needs SRTool(record_velocity=True)")
         if output_normal and self.mesh_normal is None:
             raise ValueError("output_normal needs SRTool(record_normals=True)")
         export_ply_mesh(self.mesh_vertex, self.mesh_index.reshape(-1, 3), file_path,
-                        normals=self.mesh_normal if output_normal else None)
+                        normals=self.mesh_normal if output_normal else None,
+                        velocity=self.mesh_velocity if output_velocity else None)

@@ -32,7 +32,8 @@ def _lib():
     lib.sro_create.argtypes = [P(ctypes.c_double), i32p]
     lib.sro_destroy.argtypes = [ctypes.c_void_p]
     lib.sro_run.restype = ctypes.c_int32
-    lib.sro_run.argtypes = [ctypes.c_void_p, f32p, ctypes.c_int32]
+    lib.sro_run.argtypes = [ctypes.c_void_p, f32p, f32p, ctypes.c_int32]
+    lib.sro_get_velocity.argtypes = [ctypes.c_void_p, f32p, f32p]
     lib.sro_info.argtypes = [ctypes.c_void_p, i64p]
     lib.sro_sd.restype = ctypes.c_double
     lib.sro_sd.argtypes = [ctypes.c_void_p]
@@ -58,19 +59,21 @@ class SurfaceError(RuntimeError):
 
 
 def reconstruct(pos, origin=(0.0, 0.0, 0.0), voxel_size=0.01, particle_radius=0.01, smooth_radius=0.04,
-                isolevel=0.0, anisotropic=True, smooth_iterations=1, normals=False) -> dict:
+                isolevel=0.0, anisotropic=True, smooth_iterations=1, normals=False, velocity=None) -> dict:
     """Run the oracle on positions pos (n x 3) and return every stage's output:
     grid placement, active blocks, the field before and after smoothing, the
     smoothing's sd, G and x_bar per particle, and the mesh."""
     lib = _lib()
     dp = (ctypes.c_double * 7)(*origin, voxel_size, particle_radius, smooth_radius, isolevel)
-    ip = (ctypes.c_int32 * 3)(int(bool(anisotropic)), int(smooth_iterations), int(bool(normals)))
+    ip = (ctypes.c_int32 * 4)(int(bool(anisotropic)), int(smooth_iterations), int(bool(normals)),
+                              int(velocity is not None))
     h = lib.sro_create(dp, ip)
     if not h:
         raise SurfaceError(E_ARG)
     try:
         x = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).reshape(-1, 3))
-        rc = lib.sro_run(h, _fp(x), len(x))
+        vel = None if velocity is None else np.ascontiguousarray(np.asarray(velocity, np.float32).reshape(-1, 3))
+        rc = lib.sro_run(h, _fp(x), _fp(vel) if vel is not None else None, len(x))
         if rc != 0:
             raise SurfaceError(rc)
         info = (ctypes.c_int64 * 15)()
@@ -92,7 +95,13 @@ def reconstruct(pos, origin=(0.0, 0.0, 0.0), voxel_size=0.01, particle_radius=0.
         nrm = np.empty((nv, 3), np.float32) if normals else None
         lib.sro_get_mesh(h, _fp(verts), faces.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                          _fp(nrm) if nrm is not None else None)
+        vfield = mvel = None
+        if velocity is not None:
+            vfield = np.empty(tuple(dims) + (3,), np.float32)
+            mvel = np.empty((nv, 3), np.float32)
+            lib.sro_get_velocity(h, _fp(vfield), _fp(mvel))
         return dict(lo=lo, dims=dims, active=active, n_active=nact, field0=field0, field=field,
+                    velocity_field=vfield, mesh_velocity=mvel,
                     sd=lib.sro_sd(h), G=G, xbar=xbar, vertices=verts, faces=faces, normals=nrm,
                     voxel_lo=tuple(info[9:12]), voxel_dims=tuple(info[12:15]))
     finally:

@@ -53,9 +53,10 @@ enum { SRO_OK = 0, SRO_E_ARG = -1, SRO_E_OOM = -3, SRO_E_VOXEL_OVERFLOW = -5, SR
 typedef struct sro {
   /* parameters, rounded to FP32 as the Taichi kernels see them */
   float o[3], vs, sr, iso, vol, acoef, c48, c16, lam, lam1, kr, ks, kn;
-  int aniso, iters, want_normals;
+  int aniso, iters, want_normals, want_vel;
+  float cvs3; /* voxel_size^3, contribution_coeff (:1165) */
   int n;
-  float *pos;
+  float *pos, *vel;
   /* particle hash over the voxel range of the particles */
   int vlo[3], vdim[3];
   int *vstart, *vcount, *vlist;
@@ -65,11 +66,11 @@ typedef struct sro {
   int glo[3], gdim[3], bdim[3];
   long nvert, nblock;
   unsigned char *active;
-  int64_t *acc;
-  float *field0, *field, *tmp;
+  int64_t *acc, *vacc; /* vacc: 3 per vertex */
+  float *field0, *field, *tmp, *vfield;
   double sd;
   int *cube_id;
-  float *verts, *normals;
+  float *verts, *normals, *mvel;
   int *faces;
   long nverts, ntris;
 } sro;
@@ -199,7 +200,10 @@ static inline int64_t fix_of(float v) { return (int64_t)llrint((double)v * SRO_F
 /* ---------------------------------------------------------------- lifecycle */
 
 static void sro_free_run(sro *s) {
-  free(s->pos); free(s->vstart); free(s->vcount); free(s->vlist);
+  free(s->pos); free(s->vel); free(s->vacc); free(s->vfield); free(s->mvel);
+  s->vel = s->vfield = s->mvel = NULL;
+  s->vacc = NULL;
+  free(s->vstart); free(s->vcount); free(s->vlist);
   free(s->G); free(s->xbar); free(s->detG);
   free(s->active); free(s->acc); free(s->field0); free(s->field); free(s->tmp);
   free(s->cube_id); free(s->verts); free(s->normals); free(s->faces);
@@ -238,6 +242,8 @@ sro *sro_create(const double *dp, const int *ip) {
   s->aniso = ip[0];
   s->iters = ip[1];
   s->want_normals = ip[2];
+  s->want_vel = ip[3];
+  s->cvs3 = (float)(dp[3] * dp[3] * dp[3]);
   return s;
 }
 
@@ -435,6 +441,11 @@ static int alloc_grid(sro *s) {
   s->tmp = (float *)calloc(s->nvert, sizeof(float));
   s->cube_id = (int *)calloc(s->nvert, sizeof(int));
   if (!s->active || !s->acc || !s->field0 || !s->field || !s->tmp || !s->cube_id) return SRO_E_OOM;
+  if (s->vel) {
+    s->vacc = (int64_t *)calloc(3 * (size_t)s->nvert, sizeof(int64_t));
+    s->vfield = (float *)calloc(3 * (size_t)s->nvert, sizeof(float));
+    if (!s->vacc || !s->vfield) return SRO_E_OOM;
+  }
   return SRO_OK;
 }
 
@@ -480,12 +491,23 @@ static int splat(sro *s) {
           const float w = s->vol * aniso_w(s, G, det, xb[0] - p[0], xb[1] - p[1], xb[2] - p[2]);
           if (!(fabsf(w) <= SRO_MAX_CONTRIB)) return SRO_E_RANGE;
           mark(s, x, y, z);
-          s->acc[gidx(s, x - s->glo[0], y - s->glo[1], z - s->glo[2])] += (int64_t)m * fix_of(w);
+          const long q = gidx(s, x - s->glo[0], y - s->glo[1], z - s->glo[2]);
+          s->acc[q] += (int64_t)m * fix_of(w);
+          if (s->vel) { /* generate_smooth_fluid_field_with_velocity_impl (:1160-1190) */
+            const float f = poly6(s, xi[0] - p[0], xi[1] - p[1], xi[2] - p[2], h) * s->cvs3;
+            for (int c = 0; c < 3; c++) {
+              const float u = s->vel[3 * i + c] * f;
+              if (!(fabsf(u) <= SRO_MAX_CONTRIB)) return SRO_E_RANGE;
+              s->vacc[3 * q + c] += (int64_t)m * fix_of(u);
+            }
+          }
         }
       }
     }
   }
   for (long k = 0; k < s->nvert; k++) s->field0[k] = (float)((double)s->acc[k] * SRO_FIX_INV);
+  if (s->vel)
+    for (long k = 0; k < 3 * s->nvert; k++) s->vfield[k] = (float)((double)s->vacc[k] * SRO_FIX_INV);
   /* inactive blocks read 0 whatever the accumulator holds (it holds 0) */
   return SRO_OK;
 }
@@ -727,6 +749,38 @@ void sro_qef(const float *rows, int cnt, float *x) {
 static const int CORNER[8][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0}, {0, 1, 1}, {1, 1, 1}, {1, 0, 1}, {0, 0, 1}};
 static const int EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
+/* get_velocity_at (:1352-1359): 0 outside written blocks */
+static void vel_at(const sro *s, int x, int y, int z, float *out) {
+  const int gx = x - s->glo[0], gy = y - s->glo[1], gz = z - s->glo[2];
+  if (gx < 0 || gy < 0 || gz < 0 || gx >= s->gdim[0] || gy >= s->gdim[1] || gz >= s->gdim[2] ||
+      !s->active[bidx(s, gx, gy, gz)]) {
+    out[0] = out[1] = out[2] = 0.0f;
+    return;
+  }
+  const long q = gidx(s, gx, gy, gz);
+  for (int c = 0; c < 3; c++) out[c] = s->vfield[3 * q + c];
+}
+
+/* process_velocity_at (:1362-1375) with trilinear_interpolate (:1324-1333) */
+static void velocity_at(const sro *s, const float *p, float *out) {
+  int v[3];
+  voxel_of(s, p, v);
+  float base[3];
+  vpos(s, v[0], v[1], v[2], base);
+  const float xd = (p[0] - base[0]) / s->vs, yd = (p[1] - base[1]) / s->vs, zd = (p[2] - base[2]) / s->vs;
+  float c[8][3];
+  for (int k = 0; k < 8; k++) vel_at(s, v[0] + (k >> 2), v[1] + ((k >> 1) & 1), v[2] + (k & 1), c[k]);
+  for (int a = 0; a < 3; a++) {
+    const float c00 = c[0][a] * (1.0f - xd) + c[4][a] * xd;
+    const float c01 = c[1][a] * (1.0f - xd) + c[5][a] * xd;
+    const float c10 = c[2][a] * (1.0f - xd) + c[6][a] * xd;
+    const float c11 = c[3][a] * (1.0f - xd) + c[7][a] * xd;
+    const float c0 = c00 * (1.0f - yd) + c10 * yd;
+    const float c1 = c01 * (1.0f - yd) + c11 * yd;
+    out[a] = c0 * (1.0f - zd) + c1 * zd;
+  }
+}
+
 /* vertex_interpolate (:1254-1262) */
 static inline void vinterp(float iso, const float *p1, const float *p2, float v1, float v2, float *out) {
   const float delta = v2 - v1;
@@ -865,16 +919,26 @@ static int contour(sro *s) {
     s->normals = (float *)malloc(sizeof(float) * 3 * (size_t)(nv > 0 ? nv : 1));
     for (long v = 0; v < nv; v++) normal_at(s, s->verts + 3 * v, s->normals + 3 * v);
   }
+  if (s->want_vel) { /* process_velocity (:1378-1381) */
+    s->mvel = (float *)calloc(3 * (size_t)(nv > 0 ? nv : 1), sizeof(float));
+    if (s->vfield)
+      for (long v = 0; v < nv; v++) velocity_at(s, s->verts + 3 * v, s->mvel + 3 * v);
+  }
   return SRO_OK;
 }
 
-/* SRTool.dual_contouring (:1264-1304) on positions pos[3 n] */
-int sro_run(sro *s, const float *pos, int n) {
+/* SRTool.dual_contouring (:1264-1304) on positions pos[3 n] and, when the
+ * velocity channels are on and vel is given, velocities vel[3 n] */
+int sro_run(sro *s, const float *pos, const float *vel, int n) {
   if (!s || n < 0 || (n > 0 && !pos)) return SRO_E_ARG;
   sro_free_run(s);
   s->n = n;
   s->pos = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
   if (n > 0) memcpy(s->pos, pos, sizeof(float) * 3 * (size_t)n);
+  if (s->want_vel && vel && s->aniso && n > 0) {
+    s->vel = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+    memcpy(s->vel, vel, sizeof(float) * 3 * (size_t)n);
+  }
   if (n == 0) return SRO_E_ARG;
   int rc = build_hash(s);
   if (rc) return rc;
@@ -916,6 +980,14 @@ void sro_get_aniso(const sro *s, float *G, float *xbar) {
   memcpy(G, s->G, sizeof(float) * 9 * (size_t)s->n);
   memcpy(xbar, s->xbar, sizeof(float) * 3 * (size_t)s->n);
 }
+void sro_get_velocity(const sro *s, float *vfield, float *mesh_vel) {
+  if (vfield) {
+    if (s->vfield) memcpy(vfield, s->vfield, sizeof(float) * 3 * (size_t)s->nvert);
+    else memset(vfield, 0, sizeof(float) * 3 * (size_t)s->nvert);
+  }
+  if (mesh_vel && s->mvel && s->nverts) memcpy(mesh_vel, s->mvel, sizeof(float) * 3 * (size_t)s->nverts);
+}
+
 void sro_get_mesh(const sro *s, float *verts, int *faces, float *normals) {
   if (s->nverts) memcpy(verts, s->verts, sizeof(float) * 3 * (size_t)s->nverts);
   if (s->ntris) memcpy(faces, s->faces, sizeof(int) * 3 * (size_t)s->ntris);

@@ -169,3 +169,23 @@ def test_errors_match_the_oracle(case):
     o = O.reconstruct(torus(), voxel_size=0.01, particle_radius=0.01, smooth_radius=0.03)
     assert_same(s, info, o)
     s.close()
+
+
+def test_velocity_channels_bit_exact_and_exported(tmp_path):
+    x = fluid_block(seed=5)
+    rng = np.random.Generator(np.random.PCG64(4))
+    u = rng.normal(size=x.shape).astype(np.float32)
+    kw = dict(voxel_size=0.01, particle_radius=0.02, smooth_radius=0.05)
+    s = Surface(velocity=True, normals=True, **kw)
+    info = s.run(x, u)
+    o = O.reconstruct(x, normals=True, velocity=u, **kw)
+    assert_same(s, info, o, normals=True)
+    assert np.array_equal(s.mesh_velocity(), o["mesh_velocity"])
+    s.close()
+    sr = SRTool([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], voxel_size=0.01, particle_radius=0.02, record_velocity=True)
+    sr.dual_contouring(isolevel=0, smooth_radius=0.05, num_particles=len(x), pos=x, velocity=u)
+    assert np.array_equal(sr.mesh_velocity, o["mesh_velocity"])
+    path = str(tmp_path / "vel.ply")
+    sr.export_mesh(path, output_velocity=True)
+    back = ply.read_vertices(path)
+    assert np.array_equal(np.stack([back["vx"], back["vy"], back["vz"]], 1), o["mesh_velocity"])

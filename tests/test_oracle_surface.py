@@ -338,3 +338,37 @@ def test_ply_mesh_export_round_trip(tmp_path):
     body = raw[raw.index(b"end_header\n") + len(b"end_header\n"):]
     rec = np.frombuffer(body[len(out["vertices"]) * 24:], dtype=[("n", "u1"), ("i", "<i4", (3,))])
     assert np.all(rec["n"] == 3) and np.array_equal(rec["i"], out["faces"])
+
+
+def test_velocity_channel_single_particle_and_linearity():
+    """record_velocity (:1160-1190): an isolated particle's velocity lands on its
+    anisotropic box as m * u * poly6(x - p) * voxel^3 (multiplicities by brute
+    force); doubling every velocity doubles the vertex and mesh velocities
+    and leaves the mesh alone; zero velocities give zero channels."""
+    vs, r, h = 0.01, 0.02, 0.015
+    x = np.array([[0.2013, 0.2521, 0.2492], [0.4517, 0.2488, 0.2513]], np.float32)
+    u = np.array([[0.0, 0.0, 0.0], [0.7, -1.3, 2.1]], np.float32)
+    out = S.reconstruct(x, voxel_size=vs, particle_radius=r, smooth_radius=h, smooth_iterations=0, velocity=u)
+    R = int(np.ceil(F(4.0) * F(h) / F(vs))) + 1
+    off = R // 2
+    lo, dims = out["lo"], out["dims"]
+    ix, p = _vertex_pos(lo, dims, vs)
+    c = np.floor(x[1] / vs).astype(int)
+    m = _brute_multiplicity(c, off, R, lo, dims)
+    w = _poly6(np.linalg.norm(p - x[1].astype(np.float64), axis=-1), h) * vs**3
+    expect = (m * w)[..., None] * u[1].astype(np.float64)
+    got = out["velocity_field"].astype(np.float64)
+    assert np.max(np.abs(got - expect)) <= 1e-5 * np.max(np.abs(expect))
+    xs = torus()
+    rng = np.random.Generator(np.random.PCG64(9))
+    us = rng.normal(size=xs.shape).astype(np.float32)
+    a = torus_run(velocity=us)
+    b = torus_run(velocity=2 * us)
+    assert np.array_equal(a["vertices"], b["vertices"]) and np.array_equal(a["faces"], b["faces"])
+    # linear up to the fixed-point rounding of terms below 2^-9 (2^-32 each)
+    for key in ("velocity_field", "mesh_velocity"):
+        scale = np.max(np.abs(b[key]))
+        assert np.max(np.abs(2.0 * a[key].astype(np.float64) - b[key])) <= 1e-6 * scale, key
+    assert np.any(a["mesh_velocity"] != 0)
+    z = torus_run(velocity=np.zeros_like(us))
+    assert not np.any(z["mesh_velocity"])

@@ -11,8 +11,8 @@
  *    NN_W_NOCONV (positive) = a solver hit its iteration limit.
  *    nn_last_error() gives the text of the last error on the calling thread.
  *  - The library owns all device memory; it allocates only in nn_create,
- *    nn_upload and nn_pool_* admission/tick.  Host pointers are borrowed for the
- *    duration of the call.
+ *    nn_upload, nn_pool_* admission/tick and nn_surface_create/run*.  Host
+ *    pointers are borrowed for the duration of the call.
  *  - Internally particles are kept in cell order; every array crossing this
  *    boundary is in the caller's original particle order.
  *  - A context is not thread-safe; distinct contexts are independent.

@@ -291,11 +291,15 @@ def test_ball_is_a_closed_sphere_around_its_centre():
     radius 6 voxels) gives a sphere.  Dual contouring can give non-manifold
     edges where cubes are ambiguous (with the smooth radius at one spacing it
     does); these settings give none."""
-    out = S.reconstruct(ball(), voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06)
+    out = S.reconstruct(ball(), voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06, normals=True)
     v, f = out["vertices"].astype(np.float64), out["faces"]
     assert check_closed_oriented(v, f) == 2
     r = np.linalg.norm(v - 0.5, axis=1)
     assert r.max() < 0.12 and r.min() > 0.08
+    # calc_vertex_normal is f(x - 1) - f(x + 1), minus the gradient: with the
+    # field negative inside, every interpolated normal points to the centre
+    radial = (v - 0.5) / r[:, None]
+    assert np.all(np.einsum("ij,ij->i", out["normals"], radial) < -0.8)
     in_own_cube_in_grid_order(out["vertices"], 0.01)
 
 

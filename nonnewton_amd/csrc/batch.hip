@@ -216,7 +216,7 @@ void rigid_rest_centres(SceneHost& s, const float* x) {
 }
 
 // bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wd0,x0,vdiff,vdiff_alt (9 float4),
-// wx,wr,wu,ww,wp,ws,pos3 (7 float3), wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int),
+// wx,wr,ww,wp,ws (5 float3), xu (2 float3), wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int),
 // rho,alpha,src,mu,gam,eta (6 float), nbr and spair (60 int + 60 float), partials (16 doubles per 64
 // slots), wave table
 static constexpr long long BYTES_PER_SLOT = 9 * 16 + 7 * 12 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 3;
@@ -504,11 +504,10 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.n_active, 4);
   A(nd.wx, Pc);
   A(nd.wr, Pc);
-  A(nd.wu, Pc);
   A(nd.ww, Pc);
   A(nd.wp, Pc);
   A(nd.ws, Pc);
-  A(nd.pos3, Pc);
+  A(nd.xu, Pc);
   A(nd.vdiff_alt, Pc);
   A(nd.wd0, Pc);
   A(nd.wd1, Pc);

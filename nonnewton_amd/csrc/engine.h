@@ -104,6 +104,11 @@ struct SceneState {
 };
 
 // Device pointers of a batch (passed by value to every kernel).
+struct XU {
+  float3 p;
+  float3 u;
+};
+
 struct Dev {
   int P;        // particle slots allocated (multiple of 64; the stride of per-pair arrays)
   int W;        // waves allocated = P / 64
@@ -151,11 +156,13 @@ struct Dev {
   // matvec gathers u and the position of every neighbor)
   float3* wx;  // iterate x
   float3* wr;  // residual r
-  float3* wu;  // preconditioned residual u = M r
   float3* ww;  // w = A u
   float3* wp;  // direction p
   float3* ws;  // s = A p
-  float3* pos3;  // positions during the solve (written by cg_init for every particle of a Weiler scene)
+  // what the matvec gathers of a neighbor, in one 24-byte record: its position
+  // during the solve (written by cg_init for every particle of a Weiler scene)
+  // and u = M r, the preconditioned residual (fluid particles)
+  XU* xu;
   float4* wd0;  // inverse diagonal block: (d00, d01, d02, d11)
   float2* wd1;  //                          (d12, d22)
   float* spair; // per-pair scalar s_ij of the Weiler operator, at the list word of the pair (nb_word)

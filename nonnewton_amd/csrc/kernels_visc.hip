@@ -166,7 +166,7 @@ __global__ void k_cg_init(Dev d) {
   // the positions the CG gathers, for every particle of the scene
   if (q == 0 && i < sc.p_begin + sc.n) {
     const float4 p = valid ? xi : d.pos[i];
-    d.pos3[i] = make_float3(p.x, p.y, p.z);
+    d.xu[i].p = make_float3(p.x, p.y, p.z);
   }
   if (valid && q == 0) {
     const float f = sc.dt / d.rho[i];
@@ -226,7 +226,7 @@ __global__ void k_cg_guess(Dev d) {
     const float uz = w0.z * r.x + w1.x * r.y + w1.y * r.z;
     d.wx[i] = x;
     d.wr[i] = r;
-    d.wu[i] = make_float3(ux, uy, uz);
+    d.xu[i].u = make_float3(ux, uy, uz);
     v[0] = (double)b.x * (double)b.x + (double)b.y * (double)b.y + (double)b.z * (double)b.z;
     v[1] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[2] = (double)r.x * (double)ux + (double)r.y * (double)uy + (double)r.z * (double)uz;
@@ -238,7 +238,7 @@ __global__ void k_cg_guess(Dev d) {
 // (start: after cg_guess, every Weiler scene; read by the start reduction --
 // otherwise: an iteration, active scenes; read by the next update).  D list
 // positions of a lane per batch: their list words and pair scalars, then their
-// neighbors' positions (pos3) and u, are loaded before any of them is used (a
+// neighbors' records (position and u, 24 bytes), are loaded before any of them is used (a
 // shorter chain of dependent loads); the terms are added in list order
 // whatever D is.  A pair with s_ij = 0 -- every neighbor that is not fluid --
 // adds nothing and is skipped, so the neighbor's u is never read there.
@@ -254,8 +254,9 @@ __global__ void k_cg_matvec(Dev d, int start) {
   float ax = 0.f, ay = 0.f, az = 0.f;
   float3 ui = make_float3(0.f, 0.f, 0.f);
   if (valid) {
-    const float3 xi = d.pos3[i];
-    ui = d.wu[i];
+    const XU own = d.xu[i];
+    const float3 xi = own.p;
+    ui = own.u;
     const int cnt = d.nnbr[i];
     const long P = d.P;
     for (int k = q; k < cnt; k += 4 * D) {
@@ -270,8 +271,9 @@ __global__ void k_cg_matvec(Dev d, int start) {
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
-        xj[e] = d.pos3[j[e]];
-        uj[e] = d.wu[j[e]];
+        const XU nb = d.xu[j[e]];
+        xj[e] = nb.p;
+        uj[e] = nb.u;
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
@@ -318,8 +320,9 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
     float ax = 0.f, ay = 0.f, az = 0.f;
     float3 ui = make_float3(0.f, 0.f, 0.f);
     if (valid) {
-      const float3 xi = d.pos3[i];
-      ui = d.wu[i];
+      const XU own = d.xu[i];
+      const float3 xi = own.p;
+      ui = own.u;
       const int cnt = d.nnbr[i];
       const long P = d.P;
       for (int kk = q; kk < cnt; kk += 4 * D) {
@@ -334,8 +337,9 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {
-          xj[e] = d.pos3[j[e]];
-          uj[e] = d.wu[j[e]];
+          const XU nb = d.xu[j[e]];
+          xj[e] = nb.p;
+          uj[e] = nb.u;
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {
@@ -461,7 +465,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
   const int i = w * 64 + (t & 63);
   double v[2] = {0.0, 0.0};
   if (i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID) {
-    const float3 u = d.wu[i];
+    const float3 u = d.xu[i].u;
     const float3 ww = d.ww[i];
     float3 p, sv;
     if (first) {
@@ -490,7 +494,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
     d.ws[i] = sv;
     d.wx[i] = x;
     d.wr[i] = r;
-    d.wu[i] = make_float3(zx, zy, zz);
+    d.xu[i].u = make_float3(zx, zy, zz);
     v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y + (double)r.z * (double)r.z;
     v[1] = (double)r.x * (double)zx + (double)r.y * (double)zy + (double)r.z * (double)zz;
   }

@@ -217,7 +217,7 @@ void rigid_rest_centres(SceneHost& s, const float* x) {
 
 // bytes per particle slot: pos,pos_alt,vel,vel_alt,acc,wd0,x0,vdiff,vdiff_alt (9 float4),
 // wx,wr,ww,wp,ws (5 float3), xu (2 float3), wd1 (float2), info,info_alt,key,rank,inv,bucket,nnbr,s_of (8 int),
-// rho,alpha,src,mu,gam,eta (6 float), nbr and spair (60 int + 60 float), partials (16 doubles per 64
+// rho,mu,gam,eta (4 float), sa (float2), nbr and spair (60 int + 60 float), partials (16 doubles per 64
 // slots), wave table
 static constexpr long long BYTES_PER_SLOT = 9 * 16 + 7 * 12 + 8 + 8 * 4 + 6 * 4 + 2 * MAX_NB * 4 + 3;
 static constexpr long long BYTES_PER_MEMBER = 4 + 4 + MAX_REST * 4 + 4 * 16 + 1;
@@ -388,8 +388,7 @@ int Batch::copy_scene_state(const Dev& nd, const SceneHost& s, const Batch* from
   HIPCHK(cp(nd.x0 + nb, od.x0 + ob, sizeof(float4) * n));
   HIPCHK(cp(nd.s_of + nb, od.s_of + ob, sizeof(int) * n));
   HIPCHK(cp(nd.rho + nb, od.rho + ob, sizeof(float) * n));
-  HIPCHK(cp(nd.alpha + nb, od.alpha + ob, sizeof(float) * n));
-  HIPCHK(cp(nd.src + nb, od.src + ob, sizeof(float) * n));
+  HIPCHK(cp(nd.sa + nb, od.sa + ob, sizeof(float2) * n));
   HIPCHK(cp(nd.acc + nb, od.acc + ob, sizeof(float4) * n));
   HIPCHK(cp(nd.gam + nb, od.gam + ob, sizeof(float) * n));
   HIPCHK(cp(nd.eta + nb, od.eta + ob, sizeof(float) * n));
@@ -484,8 +483,7 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.info_alt, Pc);
   A(nd.acc, Pc);
   A(nd.rho, Pc);
-  A(nd.alpha, Pc);
-  A(nd.src, Pc);
+  A(nd.sa, Pc);
   A(nd.nbr, (size_t)Pc * MAX_NB);
   A(nd.nnbr, Pc);
   A(nd.key, Pc);
@@ -568,8 +566,7 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   launch_fill_f4(nd.x0, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
   launch_fill_f4(nd.vdiff, Pc, make_float4(0.f, 0.f, 0.f, 0.f), stream);
   HIPCHK(hipMemsetAsync(nd.rho, 0, sizeof(float) * std::max(Pc, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.alpha, 0, sizeof(float) * std::max(Pc, 1), stream));
-  HIPCHK(hipMemsetAsync(nd.src, 0, sizeof(float) * std::max(Pc, 1), stream));
+  HIPCHK(hipMemsetAsync(nd.sa, 0, sizeof(float2) * std::max(Pc, 1), stream));
   HIPCHK(hipMemsetAsync(nd.gam, 0, sizeof(float) * std::max(Pc, 1), stream));
   HIPCHK(hipMemsetAsync(nd.eta, 0, sizeof(float) * std::max(Pc, 1), stream));
   HIPCHK(hipMemsetAsync(nd.acc, 0, sizeof(float4) * std::max(Pc, 1), stream));
@@ -1144,6 +1141,15 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
     v.resize(n);
     return hipMemcpyAsync(v.data(), p + pb, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
   };
+  // one component of the (src, alpha) pairs
+  auto slot_sa = [&](int comp, std::vector<float>& v) -> hipError_t {
+    std::vector<float2> t(n);
+    hipError_t e = hipMemcpyAsync(t.data(), d.sa + pb, sizeof(float2) * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    v.resize(n);
+    for (int k = 0; k < n; k++) v[k] = comp ? t[k].y : t[k].x;
+    return e;
+  };
   std::vector<float4> f4;
   std::vector<float> f1;
   std::vector<int> i1;
@@ -1186,12 +1192,12 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
     case NN_FIELD_DENSITY_ADV:
     case NN_FIELD_STRAIN_NORM:
     case NN_FIELD_VISCOSITY: {
-      const float* p = field == NN_FIELD_DENSITY ? d.rho
-                       : field == NN_FIELD_DFSPH_FACTOR ? d.alpha
-                       : field == NN_FIELD_DENSITY_ADV ? d.src
-                       : field == NN_FIELD_STRAIN_NORM ? d.gam
-                                                       : d.eta;
-      HIPCHK(slot_f(p, f1));
+      if (field == NN_FIELD_DFSPH_FACTOR || field == NN_FIELD_DENSITY_ADV) {
+        HIPCHK(slot_sa(field == NN_FIELD_DFSPH_FACTOR ? 1 : 0, f1));
+      } else {
+        const float* p = field == NN_FIELD_DENSITY ? d.rho : field == NN_FIELD_STRAIN_NORM ? d.gam : d.eta;
+        HIPCHK(slot_f(p, f1));
+      }
       HIPCHK(hipStreamSynchronize(stream));
       const SceneDev& c = s.dev;
       for (int t = 0; t < n; t++) {

@@ -136,8 +136,9 @@ struct Dev {
   int* info_alt;
   float4* acc;
   float* rho;
-  float* alpha;
-  float* src;  // density_adv of the reference (rho-dot in the divergence solve)
+  // (src, alpha): density_adv of the reference (rho-dot in the divergence
+  // solve) and the DFSPH factor, side by side: the solver updates gather both
+  float2* sa;
   int* nbr;
   int* nnbr;
   int* key;

@@ -81,10 +81,10 @@ __global__ void k_density_alpha(Dev d) {
     d.rho[i] = rho;
     rho_out = rho;
     sk += gix * gix + giy * giy + giz * giz;
-    d.alpha[i] = sk > 1e-6f ? -1.0f / sk : 0.0f;
+    const float al = sk > 1e-6f ? -1.0f / sk : 0.0f;
     rd = rd > 0.0f ? rd : 0.0f;
     if (cnt < 20) rd = 0.0f;
-    d.src[i] = rd;
+    d.sa[i] = make_float2(rd, al);
   }
   const float m = wave_max(rho_out);
   if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(&d.state[s].max_rho_bits, __float_as_uint(m));
@@ -134,11 +134,12 @@ __global__ void k_div_update(Dev d) {
   if (!d.state[s].active_v) return;
   if (i >= sc.p_begin + sc.n || kind_of(d.info[i]) != KIND_FLUID) return;
   const float4 xi = d.pos[i];
-  const float ki = d.src[i] * alpha_div(sc, d.alpha[i]);
+  const float2 sai = d.sa[i];
+  const float ki = sai.x * alpha_div(sc, sai.y);
   const int cnt = d.nnbr[i];
   float dvx = 0.f, dvy = 0.f, dvz = 0.f;
   for_neighbors<B>(
-      d, i, cnt, [&](int j) { return PosSrc{d.pos[j], d.src[j], d.alpha[j]}; },
+      d, i, cnt, [&](int j) { const float2 t = d.sa[j]; return PosSrc{d.pos[j], t.x, t.y}; },
       [&](int, const PosSrc& nj) {
         const float4 xj = nj.x;
         if (xj.w > 0.0f) {
@@ -203,7 +204,7 @@ __global__ void k_div_source(Dev d) {
     float rd = velocity_divergence<B>(d, sc, i, d.pos[i], cnt);
     rd = rd > 0.0f ? rd : 0.0f;
     if (cnt < 20) rd = 0.0f;
-    d.src[i] = rd;
+    d.sa[i].x = rd;
     v[0] = (double)sc.rho0 * (double)rd - 0.0;
   }
   wave_partials<1>(d, (int)(i >> 6), v);
@@ -350,7 +351,7 @@ __global__ void k_press_source(Dev d, int reduce) {
     const float delta = velocity_divergence<B>(d, sc, i, d.pos[i], cnt);
     float da = d.rho[i] / sc.rho0 + sc.dt * delta;
     da = da > 1.0f ? da : 1.0f;
-    d.src[i] = da;
+    d.sa[i].x = da;
     v[0] = (double)sc.rho0 * (double)da - sc.rho0_d;
   }
   if (reduce) wave_partials<1>(d, (int)(i >> 6), v);
@@ -466,10 +467,11 @@ __global__ void k_press_update(Dev d) {
   const float4 xi = d.pos[i];
   const int cnt = d.nnbr[i];
   if (kind == KIND_FLUID) {
-    const float ki = (d.src[i] - 1.0f) * alpha_press(sc, d.alpha[i]);
+    const float2 sai = d.sa[i];
+    const float ki = (sai.x - 1.0f) * alpha_press(sc, sai.y);
     float4 v = d.vel[i];
     for_neighbors<B>(
-        d, i, cnt, [&](int j) { return PosSrc{d.pos[j], d.src[j], d.alpha[j]}; },
+        d, i, cnt, [&](int j) { const float2 t = d.sa[j]; return PosSrc{d.pos[j], t.x, t.y}; },
         [&](int, const PosSrc& nj) {
           const float4 xj = nj.x;
           if (xj.w > 0.0f) {
@@ -502,7 +504,8 @@ __global__ void k_press_update(Dev d) {
       const int f = d.nbr[nb_word(P, i, k)];
       const float4 xf = d.pos[f];
       if (!(xf.w > 0.0f)) continue;
-      const float kf = (d.src[f] - 1.0f) * alpha_press(sc, d.alpha[f]);
+      const float2 saf = d.sa[f];
+      const float kf = (saf.x - 1.0f) * alpha_press(sc, saf.y);
       if (fabsf(kf) > sc.eps) {
         const float rx = xf.x - xi.x, ry = xf.y - xi.y, rz = xf.z - xi.z;
         const float3 g = kern_grad(sc.h, sc.kG, rx, ry, rz, norm3(rx, ry, rz));

@@ -68,3 +68,38 @@ def test_product_does_not_import_the_oracle():
     for path in glob.glob(os.path.join(ROOT, "nonnewton_amd", "**", "*.py"), recursive=True):
         text = open(path).read()
         assert "oracle" not in text.replace("the oracle", ""), path
+
+
+def test_surface_argument_errors_without_gpu():
+    """nn_surface_create rejects bad parameters before it touches a device."""
+    import ctypes
+
+    lib = L.load()
+    h = ctypes.c_void_p()
+    assert lib.nn_surface_create(None, ctypes.byref(h)) == L.NN_E_ARG
+    for field, value in (("voxel_size", 0.0), ("particle_radius", -1.0), ("smooth_radius", 0.0),
+                         ("smooth_iterations", -1), ("smooth_iterations", 65)):
+        p = L.NnSurfaceParams()
+        p.voxel_size, p.particle_radius, p.smooth_radius, p.smooth_iterations = 0.01, 0.02, 0.06, 1
+        setattr(p, field, value)
+        assert lib.nn_surface_create(ctypes.byref(p), ctypes.byref(h)) == L.NN_E_ARG, field
+    p = L.NnSurfaceParams()
+    p.voxel_size, p.particle_radius, p.smooth_radius, p.anisotropic = 0.01, 0.02, 0.2, 1  # box of 82 vertices
+    assert lib.nn_surface_create(ctypes.byref(p), ctypes.byref(h)) == L.NN_E_UNSUPPORTED
+    assert "32" in L.last_error()
+    assert lib.nn_surface_run(None, None, 0) == L.NN_E_ARG
+
+
+def test_surface_params_layout_matches_header(tmp_path):
+    fields = [f for f, _ in L.NnSurfaceParams._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "nnsph.h"', "int main(void){",
+           'printf("%zu\\n", sizeof(nn_surface_params));']
+    src += [f'printf("%zu\\n", offsetof(nn_surface_params, {f}));' for f in fields]
+    src += ["return 0;}"]
+    c = tmp_path / "layout_sr.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout_sr"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)])
+    out = [int(v) for v in subprocess.check_output([str(exe)]).decode().split()]
+    assert out[0] == ctypes.sizeof(L.NnSurfaceParams)
+    assert out[1:] == [getattr(L.NnSurfaceParams, f).offset for f in fields]

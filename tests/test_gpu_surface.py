@@ -189,3 +189,15 @@ def test_velocity_channels_bit_exact_and_exported(tmp_path):
     sr.export_mesh(path, output_velocity=True)
     back = ply.read_vertices(path)
     assert np.array_equal(np.stack([back["vx"], back["vy"], back["vz"]], 1), o["mesh_velocity"])
+
+
+def test_budget_caps_the_grid():
+    s = Surface(voxel_size=0.01, particle_radius=0.02, smooth_radius=0.06)
+    s.set_budget(1 << 20)
+    with pytest.raises(L.NNError) as e:
+        s.run(torus())
+    assert e.value.code == L.NN_E_OOM
+    s.set_budget(0)
+    info = s.run(torus())
+    assert info["num_triangles"] > 0
+    s.close()

@@ -244,13 +244,10 @@ __global__ void k_nonpressure(Dev d) {
         float rho;
       };
       // surface tension reads positions only: the velocities and densities
-      // are gathered only for a viscosity term formed here
+      // are gathered only for a viscosity term formed here (two loops, so
+      // that the position-only one holds no velocity registers)
       const bool need_v = expl || bnu_on;
-      const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-      for_neighbors<B>(
-          d, i, cnt,
-          [&](int j) { return need_v ? NJ{d.pos[j], d.vel[j], d.rho[j]} : NJ{d.pos[j], zero4, 0.0f}; },
-          [&](int, const NJ& nj) {
+      auto body = [&](int, const NJ& nj) {
             const float4 xj = nj.x, vj = nj.v;
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             if (xj.w > 0.0f) {
@@ -279,7 +276,11 @@ __global__ void k_nonpressure(Dev d) {
               wvy += f * g.y;
               wvz += f * g.z;
             }
-          });
+          };
+      if (need_v)
+        for_neighbors<B>(d, i, cnt, [&](int j) { return NJ{d.pos[j], d.vel[j], d.rho[j]}; }, body);
+      else
+        for_neighbors<B>(d, i, cnt, [&](int j) { return NJ{d.pos[j], make_float4(0.f, 0.f, 0.f, 0.f), 0.0f}; }, body);
     }
     float ax = sc.gx, ay = sc.gy, az = sc.gz;
     if (st_on) {

@@ -1289,7 +1289,8 @@ static int run(Surface& S, const float* dpos, const float* dvel, long n) {
   k.nvert = nvert;
   k.nblock = nblock;
   const double need = (double)nvox * 8 + (double)nvert * (k.vel ? 72 : 36) + (double)n * 64 + (double)nblock * 20;
-  if (nvert > (1L << 31) - 2 || nvox > (1L << 31) - 2 || (S.budget > 0 && need > (double)S.budget)) {
+  // int32 scans: up to 6 triangles per cube must stay below 2^31
+  if (nvert > (1L << 28) || nvox > (1L << 30) || (S.budget > 0 && need > (double)S.budget)) {
     set_error("nn_surface_run: the vertex grid over the particles' extent exceeds the device budget (" +
               std::to_string(nvert) + " vertices)");
     return NN_E_OOM;

@@ -154,6 +154,8 @@ def load():
         "nn_pool_footprint": (i64, [i64, i64, P(NnParams)]),
         "nn_pool_admit": (i32, [vp, P(NnParams), P(ctypes.c_float), P(ctypes.c_float), P(i32), i64, P(i32)]),
         "nn_pool_tick": (i32, [vp, P(NnStepStats), i32, P(i32)]),
+        "nn_pool_tick_async": (i32, [vp]),
+        "nn_pool_sync": (i32, [vp, P(NnStepStats), i32, P(i32)]),
         "nn_pool_retire": (i32, [vp, i32]),
         "nn_pool_steps": (i64, [vp, i32]),
         "nn_pool_read": (i32, [vp, i32, i32, vp, sz]),
@@ -403,6 +405,18 @@ class Pool:
         rc = self.lib.nn_pool_tick(self.h, self._stats, self.capacity, ctypes.byref(stepped))
         if rc not in self.SCENE_ERRORS:
             check(rc, "nn_pool_tick")
+        return rc, int(stepped.value)
+
+    def tick_async(self):
+        """Start a tick on a library thread and return at once (sync() waits)."""
+        check(self.lib.nn_pool_tick_async(self.h), "nn_pool_tick_async")
+
+    def sync(self):
+        """Wait for the tick tick_async started: (rc, scenes stepped), as tick()."""
+        stepped = ctypes.c_int32(0)
+        rc = self.lib.nn_pool_sync(self.h, self._stats, self.capacity, ctypes.byref(stepped))
+        if rc not in self.SCENE_ERRORS:
+            check(rc, "nn_pool_sync")
         return rc, int(stepped.value)
 
     def stats(self, slot: int) -> dict:

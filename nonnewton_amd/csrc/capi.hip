@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 
 #include "batch.h"
 
@@ -334,6 +335,12 @@ struct nn_pool {
   // (hipFree waits for the device, so not on the thread that ticks)
   std::vector<std::unique_ptr<Batch>> graveyard;
   std::vector<void*> dead_arrays;
+  // a tick in flight (nn_pool_tick_async): its thread, and what nn_pool_sync returns
+  std::thread async;
+  bool async_on = false;
+  int async_rc = 0, async_stepped = 0;
+  std::string async_err;
+  std::vector<nn_step_stats> async_stats;
 };
 
 // frees what the ticks left behind; called outside the pool lock
@@ -372,6 +379,10 @@ int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) {
 
 int32_t nn_pool_destroy(nn_pool* p) {
   if (!p) return NN_E_ARG;
+  if (p->async_on) {  // a tick in flight finishes first
+    p->async.join();
+    p->async_on = false;
+  }
   if (p->main) p->main->take_deferred(p->dead_arrays);
   pool_free_dead(p);
   p->main.reset();
@@ -541,8 +552,7 @@ static int pool_apply_membership(nn_pool* p) {
   return NN_OK;
 }
 
-int32_t nn_pool_tick(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_t* stepped) {
-  if (!p) return NN_E_ARG;
+static int pool_tick_impl(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_t* stepped) {
   std::lock_guard<std::mutex> lk(p->mu);
   int rc = pool_apply_membership(p);
   if (rc < 0) return rc;
@@ -556,6 +566,57 @@ int32_t nn_pool_tick(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_
     if (stats && s < stats_len) stats[s] = st[k];
   }
   return rc;
+}
+
+int32_t nn_pool_tick(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_t* stepped) {
+  if (!p) return NN_E_ARG;
+  if (p->async_on) {
+    nn::set_error("nn_pool_tick: a tick is in flight (nn_pool_sync first)");
+    return NN_E_BUSY;
+  }
+  return pool_tick_impl(p, stats, stats_len, stepped);
+}
+
+int32_t nn_pool_tick_async(nn_pool* p) {
+  if (!p) return NN_E_ARG;
+  if (p->async_on) {
+    nn::set_error("nn_pool_tick_async: a tick is already in flight (nn_pool_sync first)");
+    return NN_E_BUSY;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nn::hip_fail(e, "nn_pool_tick_async");
+  nn_step_stats unset{};
+  unset.iterations_v = -1;  // marks the slots the tick does not step
+  p->async_stats.assign(POOL_CAPACITY, unset);
+  p->async_err.clear();
+  p->async_on = true;
+  p->async = std::thread([p, dev] {
+    int stepped = 0;
+    hipError_t se = hipSetDevice(dev);
+    const int rc = se != hipSuccess ? nn::hip_fail(se, "nn_pool_tick_async: device")
+                                    : pool_tick_impl(p, p->async_stats.data(), POOL_CAPACITY, &stepped);
+    if (rc != NN_OK) p->async_err = nn::last_error_text();
+    p->async_stepped = stepped;
+    p->async_rc = rc;
+  });
+  return NN_OK;
+}
+
+int32_t nn_pool_sync(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_t* stepped) {
+  if (!p) return NN_E_ARG;
+  if (!p->async_on) {
+    nn::set_error("nn_pool_sync: no tick in flight");
+    return NN_E_STATE;
+  }
+  p->async.join();
+  p->async_on = false;
+  if (stats)
+    for (int s = 0; s < std::min<int>(stats_len, POOL_CAPACITY); s++)
+      if (p->async_stats[s].iterations_v >= 0) stats[s] = p->async_stats[s];
+  if (stepped) *stepped = p->async_stepped;
+  if (p->async_rc != NN_OK) nn::set_error(p->async_err);
+  return p->async_rc;
 }
 
 int64_t nn_pool_steps(nn_pool* p, int32_t slot) {

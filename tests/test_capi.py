@@ -103,3 +103,9 @@ def test_surface_params_layout_matches_header(tmp_path):
     out = [int(v) for v in subprocess.check_output([str(exe)]).decode().split()]
     assert out[0] == ctypes.sizeof(L.NnSurfaceParams)
     assert out[1:] == [getattr(L.NnSurfaceParams, f).offset for f in fields]
+
+
+def test_async_tick_argument_errors_without_gpu():
+    lib = L.load()
+    assert lib.nn_pool_tick_async(None) == L.NN_E_ARG
+    assert lib.nn_pool_sync(None, None, 0, None) == L.NN_E_ARG

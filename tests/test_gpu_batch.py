@@ -179,3 +179,38 @@ def test_late_cg_iterations_on_the_lists():
             assert np.array_equal(pool.read(slot, f), a), (sc.name, f)
         assert pool.stats(slot)["iterations_cg"] == st["iterations_cg"]
     pool.close()
+
+
+def test_async_tick_equals_sync_tick():
+    """nn_pool_tick_async + nn_pool_sync step the pool exactly as nn_pool_tick
+    does (same fields, same stats), a second start while one is in flight is
+    NN_E_BUSY, as is a synchronous tick, and a sync without a tick is
+    NN_E_STATE.  Admission while a tick is in flight waits for it."""
+    batch = scenes.mixed_batch(count=3, lo=2000, hi=5000)
+    pools = [L.Pool(8 << 30), L.Pool(8 << 30)]
+    slots = [[p.admit(s.params, s.x, s.kind, s.v) for s in batch[:2]] for p in pools]
+    for step in range(4):
+        rc, n = pools[0].tick()
+        pools[1].tick_async()
+        if step == 1:
+            with pytest.raises(L.NNError) as e:
+                pools[1].tick_async()
+            assert e.value.code == L.NN_E_BUSY
+            r2 = pools[1].lib.nn_pool_tick(pools[1].h, None, 0, None)
+            assert r2 == L.NN_E_BUSY
+        if step == 2:  # admission overlaps the running tick
+            slots[1].append(pools[1].admit(batch[2].params, batch[2].x, batch[2].kind, batch[2].v))
+        rc2, n2 = pools[1].sync()
+        if step == 2:
+            slots[0].append(pools[0].admit(batch[2].params, batch[2].x, batch[2].kind, batch[2].v))
+        assert (rc, n) == (rc2, n2)
+        for a, b in zip(slots[0], slots[1]):
+            assert pools[0].stats(a) == pools[1].stats(b)
+    for a, b in zip(slots[0], slots[1]):
+        for f in (L.FIELD_X, L.FIELD_V, L.FIELD_DENSITY):
+            assert np.array_equal(pools[0].read(a, f), pools[1].read(b, f))
+    with pytest.raises(L.NNError) as e:
+        pools[1].sync()
+    assert e.value.code == L.NN_E_STATE
+    for p in pools:
+        p.close()

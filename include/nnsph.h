@@ -285,6 +285,12 @@ int32_t nn_surface_set_budget(nn_surface *s, int64_t bytes);
 int32_t nn_surface_run(nn_surface *s, const float *pos, int64_t n);
 /* The same on positions already in device memory (n x 3 float, caller order). */
 int32_t nn_surface_run_device(nn_surface *s, const float *device_pos, int64_t n);
+/* The same on the fluid particles of a context, where its last step left
+ * them in device memory: the mesh of the simulation's current state without a
+ * round trip through the host (the reference passes meta.pd.x to SRTool).
+ * Fluid particles must be numbered 0 .. n_fluid - 1, as the reference numbers
+ * them (SPH/main.py:627); NN_E_UNSUPPORTED otherwise. */
+int32_t nn_surface_run_context(nn_surface *s, nn_ctx *ctx);
 /* dual_contouring(..., velocity) with record_velocity (params.velocity set):
  * the particles' velocities (n x 3) are splatted with the poly6 weight times
  * voxel_size^3 (generate_smooth_fluid_field_with_velocity_impl, :1160-1190) and
@@ -295,7 +301,8 @@ int32_t nn_surface_mesh_velocity(nn_surface *s, float *out);
 /* num_vertices / num_indices of the last run and its grid (srtool.py:917-918):
  * out[0..2] first grid vertex, out[3..5] grid dims (vertices), out[6] vertices,
  * out[7] triangles, out[8] written 32^3 blocks, out[9..11] first voxel of the
- * particle hash, out[12..14] its dims; *sd = the smoothing's sd (:1437-1456). */
+ * particle hash, out[12..14] its dims, out[15] the particles meshed; *sd = the
+ * smoothing's sd (:1437-1456).  out holds 16 entries. */
 int32_t nn_surface_info(nn_surface *s, int64_t *out, double *sd);
 /* mesh_vertex, mesh_index as triangles, mesh_normal (export_mesh, :1802-1838);
  * normals may be NULL, and must be NULL unless params.normals was set. */

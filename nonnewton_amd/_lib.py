@@ -185,6 +185,7 @@ def load():
         "nn_surface_field": (i32, [vp, i32, P(ctypes.c_float), sz]),
         "nn_surface_aniso": (i32, [vp, P(ctypes.c_float), P(ctypes.c_float), i64]),
         "nn_surface_run_velocity": (i32, [vp, P(ctypes.c_float), P(ctypes.c_float), i64]),
+        "nn_surface_run_context": (i32, [vp, vp]),
         "nn_surface_mesh_velocity": (i32, [vp, P(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():

@@ -14,6 +14,10 @@
 namespace nn {
 
 void set_error(const std::string& msg);
+class Batch;
+// surface reconstruction of scene k's fluid particles, as they lie in the
+// batch's device arrays (surface.hip)
+int surface_run_scene(nn_surface* s, Batch& b, int k);
 int hip_fail(hipError_t e, const char* what);
 
 struct SceneHost {

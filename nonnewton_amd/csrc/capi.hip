@@ -358,6 +358,18 @@ static void pool_free_dead(nn_pool* p) {
 
 extern "C" {
 
+int32_t nn_surface_run_context(nn_surface* s, nn_ctx* ctx) {
+  if (!s || !ctx || !ctx->batch) {
+    nn::set_error("nn_surface_run_context: bad arguments");
+    return NN_E_ARG;
+  }
+  if (!ctx->uploaded || ctx->batch->scenes.empty()) {
+    nn::set_error("nn_surface_run_context: the context holds no particles (nn_upload first)");
+    return NN_E_STATE;
+  }
+  return nn::surface_run_scene(s, *ctx->batch, 0);
+}
+
 int32_t nn_pool_capacity(void) { return POOL_CAPACITY; }
 
 int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) {

@@ -1406,6 +1406,24 @@ static int run(Surface& S, const float* dpos, const float* dvel, long n) {
   return NN_OK;
 }
 
+// a scene's fluid positions, from the batch's cell order into original-index
+// order (the reference numbers the fluid first, so ids are 0 .. n_fluid - 1)
+__global__ void k_fluid_positions(const float4* pos, const int* info, int pb, int n, int nf, float* out, int* err) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int inf = info[pb + t];
+  if ((inf >> 28) != KIND_FLUID) return;
+  const int id = inf & 0x0FFFFFFF;
+  if (id >= nf) {
+    atomicOr(err, 1);
+    return;
+  }
+  const float4 p = pos[pb + t];
+  out[3 * (long)id] = p.x;
+  out[3 * (long)id + 1] = p.y;
+  out[3 * (long)id + 2] = p.z;
+}
+
 }  // namespace sr
 }  // namespace nn
 
@@ -1414,6 +1432,32 @@ using nn::sr::Surface;
 struct nn_surface {
   Surface impl;
 };
+
+int nn::surface_run_scene(nn_surface* h, Batch& b, int k) {
+  Surface& S = h->impl;
+  const SceneHost& sc = *b.scenes[k];
+  const int nf = sc.n_fluid;
+  if (nf <= 0) {
+    set_error("nn_surface_run_context: the scene has no fluid particles");
+    return NN_E_ARG;
+  }
+  hipError_t e = hipStreamSynchronize(b.stream);  // the step's positions are complete
+  if (e == hipSuccess) e = S.pos.ensure(3 * (size_t)nf);
+  if (e == hipSuccess) e = S.err.ensure(1);
+  if (e != hipSuccess) return hip_fail(e, "nn_surface_run_context");
+  hipLaunchKernelGGL(sr::k_fill_i32, dim3(1), dim3(64), 0, S.st, S.err.p, 0, 1L);
+  hipLaunchKernelGGL(sr::k_fluid_positions, dim3((sc.n + 255) / 256), dim3(256), 0, S.st, b.d.pos, b.d.info, sc.p_begin,
+                     sc.n, nf, S.pos.p, S.err.p);
+  int err = 0;
+  e = hipMemcpyAsync(&err, S.err.p, sizeof(int), hipMemcpyDeviceToHost, S.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(S.st);
+  if (e != hipSuccess) return hip_fail(e, "nn_surface_run_context");
+  if (err) {
+    set_error("nn_surface_run_context: fluid particles must be numbered 0 .. n_fluid - 1 (the reference numbers them first)");
+    return NN_E_UNSUPPORTED;
+  }
+  return sr::run(S, S.pos.p, nullptr, nf);
+}
 
 extern "C" {
 
@@ -1559,6 +1603,7 @@ int32_t nn_surface_info(nn_surface* s, int64_t* out, double* sd) {
   out[6] = S->nverts;
   out[7] = S->ntris;
   out[8] = S->nactive;
+  out[15] = k.n;
   if (sd) *sd = S->sd;
   return NN_OK;
 }

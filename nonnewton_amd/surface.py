@@ -85,6 +85,14 @@ class Surface:
             L.check(L.load().nn_surface_run_velocity(self._h, L.fptr(x), L.fptr(u), len(x)), "nn_surface_run_velocity")
         return self.info()
 
+    def run_context(self, ctx) -> dict:
+        """Reconstruct from the fluid particles of a nonnewton_amd._lib.Context
+        as its last step left them on the device."""
+        L.check(L.load().nn_surface_run_context(self._h, ctx.h), "nn_surface_run_context")
+        info = self.info()
+        self.n = int(info["particles"])
+        return info
+
     def run_device(self, ptr: int, n: int) -> dict:
         """Reconstruct from n positions (x, y, z float triples, caller order)
         already in device memory at address ptr; the caller's writes to them
@@ -94,11 +102,12 @@ class Surface:
         return self.info()
 
     def info(self) -> dict:
-        out = (ctypes.c_int64 * 15)()
+        out = (ctypes.c_int64 * 16)()
         sd = ctypes.c_double()
         L.check(L.load().nn_surface_info(self._h, out, ctypes.byref(sd)), "nn_surface_info")
         return dict(lo=tuple(out[0:3]), dims=tuple(out[3:6]), num_vertices=out[6], num_triangles=out[7],
-                    active_blocks=out[8], voxel_lo=tuple(out[9:12]), voxel_dims=tuple(out[12:15]), sd=sd.value)
+                    active_blocks=out[8], voxel_lo=tuple(out[9:12]), voxel_dims=tuple(out[12:15]), particles=out[15],
+                    sd=sd.value)
 
     def mesh(self):
         """(vertices (V, 3), faces (T, 3), normals (V, 3) or None)."""

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from nonnewton_amd import _lib as L
-from nonnewton_amd import ply
+from nonnewton_amd import ply, scenes
 from nonnewton_amd.surface import SRTool, Surface
 from oracle import surface as O
 
@@ -201,3 +201,32 @@ def test_budget_caps_the_grid():
     info = s.run(torus())
     assert info["num_triangles"] > 0
     s.close()
+
+
+def test_mesh_of_a_context_equals_mesh_of_its_downloaded_fluid():
+    """nn_surface_run_context meshes the fluid where the step left it in device
+    memory; the mesh equals the one of the downloaded fluid positions (and the
+    oracle's), bit for bit."""
+    from nonnewton_amd.params import KIND_FLUID
+
+    sc = scenes.dam_break(fluid_counts=(16, 12, 16))
+    ctx = L.Context(sc.params, sc.n)
+    ctx.upload(sc.x, sc.kind, sc.v)
+    ctx.init()
+    for _ in range(5):
+        ctx.step()
+    x = ctx.download(L.FIELD_X).reshape(-1, 3)
+    fluid = x[sc.kind == KIND_FLUID]
+    assert np.all(np.flatnonzero(sc.kind == KIND_FLUID) == np.arange(len(fluid)))  # fluid numbered first
+    kw = dict(voxel_size=0.01, particle_radius=0.02, smooth_radius=0.05)
+    s = Surface(**kw)
+    info = s.run_context(ctx)
+    assert info["particles"] == len(fluid)
+    v1, f1, _ = s.mesh()
+    info2 = s.run(fluid)
+    v2, f2, _ = s.mesh()
+    assert np.array_equal(v1, v2) and np.array_equal(f1, f2) and len(f1) > 0
+    o = O.reconstruct(fluid, **kw)
+    assert np.array_equal(v1, o["vertices"]) and np.array_equal(f1, o["faces"])
+    s.close()
+    ctx.close()

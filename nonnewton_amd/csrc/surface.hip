@@ -619,6 +619,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_field(K k) {
     }
     __syncthreads();
     const int nst = min(GATHER_THREADS, total - s0);
+#pragma unroll 4
     for (int r = 0; r < nst; r++) {
       const int c0 = s_c[0][r], c1 = s_c[1][r], c2 = s_c[2][r];
       if (abs(vx - c0) <= 1 && abs(vy - c1) <= 1 && abs(vz - c2) <= 1) {

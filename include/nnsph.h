@@ -289,7 +289,8 @@ int32_t nn_surface_run_device(nn_surface *s, const float *device_pos, int64_t n)
  * them in device memory: the mesh of the simulation's current state without a
  * round trip through the host (the reference passes meta.pd.x to SRTool).
  * Fluid particles must be numbered 0 .. n_fluid - 1, as the reference numbers
- * them (SPH/main.py:627); NN_E_UNSUPPORTED otherwise. */
+ * them (SPH/main.py:627); NN_E_UNSUPPORTED otherwise.  With params.velocity
+ * the fluid's velocities feed the velocity channels. */
 int32_t nn_surface_run_context(nn_surface *s, nn_ctx *ctx);
 /* dual_contouring(..., velocity) with record_velocity (params.velocity set):
  * the particles' velocities (n x 3) are splatted with the poly6 weight times

@@ -1409,7 +1409,8 @@ static int run(Surface& S, const float* dpos, const float* dvel, long n) {
 
 // a scene's fluid positions, from the batch's cell order into original-index
 // order (the reference numbers the fluid first, so ids are 0 .. n_fluid - 1)
-__global__ void k_fluid_positions(const float4* pos, const int* info, int pb, int n, int nf, float* out, int* err) {
+__global__ void k_fluid_positions(const float4* pos, const float4* vel, const int* info, int pb, int n, int nf, float* out,
+                                  float* vout, int* err) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int inf = info[pb + t];
@@ -1423,6 +1424,12 @@ __global__ void k_fluid_positions(const float4* pos, const int* info, int pb, in
   out[3 * (long)id] = p.x;
   out[3 * (long)id + 1] = p.y;
   out[3 * (long)id + 2] = p.z;
+  if (vout) {
+    const float4 v = vel[pb + t];
+    vout[3 * (long)id] = v.x;
+    vout[3 * (long)id + 1] = v.y;
+    vout[3 * (long)id + 2] = v.z;
+  }
 }
 
 }  // namespace sr
@@ -1445,10 +1452,12 @@ int nn::surface_run_scene(nn_surface* h, Batch& b, int k) {
   hipError_t e = hipStreamSynchronize(b.stream);  // the step's positions are complete
   if (e == hipSuccess) e = S.pos.ensure(3 * (size_t)nf);
   if (e == hipSuccess) e = S.err.ensure(1);
+  const bool with_vel = S.prm.velocity != 0;
+  if (e == hipSuccess && with_vel) e = S.vel.ensure(3 * (size_t)nf);
   if (e != hipSuccess) return hip_fail(e, "nn_surface_run_context");
   hipLaunchKernelGGL(sr::k_fill_i32, dim3(1), dim3(64), 0, S.st, S.err.p, 0, 1L);
-  hipLaunchKernelGGL(sr::k_fluid_positions, dim3((sc.n + 255) / 256), dim3(256), 0, S.st, b.d.pos, b.d.info, sc.p_begin,
-                     sc.n, nf, S.pos.p, S.err.p);
+  hipLaunchKernelGGL(sr::k_fluid_positions, dim3((sc.n + 255) / 256), dim3(256), 0, S.st, b.d.pos, b.d.vel, b.d.info,
+                     sc.p_begin, sc.n, nf, S.pos.p, with_vel ? S.vel.p : nullptr, S.err.p);
   int err = 0;
   e = hipMemcpyAsync(&err, S.err.p, sizeof(int), hipMemcpyDeviceToHost, S.st);
   if (e == hipSuccess) e = hipStreamSynchronize(S.st);
@@ -1457,7 +1466,7 @@ int nn::surface_run_scene(nn_surface* h, Batch& b, int k) {
     set_error("nn_surface_run_context: fluid particles must be numbered 0 .. n_fluid - 1 (the reference numbers them first)");
     return NN_E_UNSUPPORTED;
   }
-  return sr::run(S, S.pos.p, nullptr, nf);
+  return sr::run(S, S.pos.p, with_vel ? S.vel.p : nullptr, nf);
 }
 
 extern "C" {

@@ -229,4 +229,11 @@ def test_mesh_of_a_context_equals_mesh_of_its_downloaded_fluid():
     o = O.reconstruct(fluid, **kw)
     assert np.array_equal(v1, o["vertices"]) and np.array_equal(f1, o["faces"])
     s.close()
+    # with the velocity channels: the fluid's velocities where the step left them
+    u = ctx.download(L.FIELD_V).reshape(-1, 3)[sc.kind == KIND_FLUID]
+    sv = Surface(velocity=True, **kw)
+    sv.run_context(ctx)
+    ov = O.reconstruct(fluid, velocity=u, **kw)
+    assert np.array_equal(sv.mesh_velocity(), ov["mesh_velocity"]) and np.any(ov["mesh_velocity"] != 0)
+    sv.close()
     ctx.close()

@@ -254,7 +254,8 @@ int32_t nn_pool_kernel_launches(nn_pool *pool, int64_t *seen, int32_t n);
  * depend on the order the GPU adds the particles' contributions; mesh vertices
  * are numbered in grid order of their cubes (x slowest, z fastest), triangles
  * in grid order of the cube that emits them.  The library allocates device
- * memory in nn_surface_run* when the grid grows.
+ * memory in nn_surface_run* when the grid grows.  An nn_surface is not
+ * thread-safe; distinct ones are independent, each on its own stream. 
  * ------------------------------------------------------------------------ */
 typedef struct nn_surface nn_surface;
 

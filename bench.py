@@ -8,7 +8,8 @@ Weiler implicit solver, 25% of scenes with an elastic block, dt 5e-4.
   value     particle-steps/s of the batched form: all 64 scenes resident in one
             pool, one launch sequence per tick (a particle-step = one dynamic
             particle advanced one step; boundary particles are not counted)
-  plain     the same scenes stepped one at a time, one context each
+  plain     the same scenes stepped one at a time, one context each, over the
+            same step indices as the batched form (same warm-up, same count)
   latency   p50/p99 scene-step latency: the tick time for the batched form,
             the per-scene step time for the plain form
   roofline  the kernel with the largest share of a profiled tick: algorithmic
@@ -204,16 +205,18 @@ def run_plain(scs, steps, warmup, block=0, shapes=None):
             c.step()
         ctxs.append(c)
     lat = []
+    its = []
     t_start = time.perf_counter()
     for k in range(steps):
         for c in ctxs:
             t = time.perf_counter()
-            c.step()
+            st = c.step()
             lat.append(time.perf_counter() - t)
+            its.append((st["iterations_v"], st["iterations_p"], st["iterations_cg"]))
     elapsed = time.perf_counter() - t_start
     for c in ctxs:
         c.close()
-    return elapsed, lat
+    return elapsed, lat, np.array(its, dtype=np.float64)
 
 
 def run_serving(count, rate):
@@ -362,7 +365,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=0, help="timed steps of the plain form (0 = --steps)")
     ap.add_argument("--workload", default="mixed_batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -407,9 +410,12 @@ def main():
     plain_value = None
     plain_lat = []
     if not args.no_plain:
-        pe, plain_lat = run_plain(scs, args.plain_steps, 1, block=args.block, shapes=shapes)
-        plain_value = n_dyn * args.plain_steps / pe
-        log(f"[bench] plain: {1e3 * pe / args.plain_steps:.3f} ms per round of {len(scs)} scenes, "
+        # the same step indices as the batched form's timed ticks (same warm-up,
+        # same count), so both forms do the same solver work
+        plain_steps = args.plain_steps or args.steps
+        pe, plain_lat, plain_its = run_plain(scs, plain_steps, args.warmup, block=args.block, shapes=shapes)
+        plain_value = n_dyn * plain_steps / pe
+        log(f"[bench] plain: {1e3 * pe / plain_steps:.3f} ms per round of {len(scs)} scenes, "
             f"{plain_value:.4g} particle-steps/s")
 
     # roofline of the dominant kernel
@@ -466,6 +472,8 @@ def main():
         cfg["batched_over_plain"] = round(value / plain_value, 3)
         cfg["p50_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 50)), 3)
         cfg["p99_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 99)), 3)
+        cfg["plain_steps_timed"] = plain_steps
+        cfg["mean_iterations_v_p_cg_plain"] = [round(float(x), 2) for x in plain_its.mean(axis=0)]
 
     if args.serve_requests > 0 and args.workload == "mixed_batch":
         cfg["serving"] = run_serving(args.serve_requests, args.serve_rate)

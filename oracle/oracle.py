@@ -28,6 +28,8 @@ F_X, F_V, F_ACC, F_VOLUME, F_DENSITY, F_DFSPH_FACTOR, F_DENSITY_ADV = 0, 1, 2, 3
 F_STRAIN_NORM, F_VISCOSITY, F_ROTATION, F_STRESS, F_F, F_REST_VOLUME, F_X0 = 7, 8, 9, 10, 11, 12, 13
 _WIDTH = {F_X: 3, F_V: 3, F_ACC: 3, F_X0: 3, F_ROTATION: 9, F_F: 9, F_STRESS: 6}
 
+PROBE_VISCOSITY, PROBE_SURFACE_TENSION, PROBE_ELASTIC = 0, 1, 2
+
 ST_NEIGHBOR_OVERFLOW, ST_CELL_OVERFLOW, ST_OUT_OF_GRID, ST_NAN, ST_REST_OVERFLOW = 1, 2, 4, 8, 16
 
 
@@ -60,7 +62,9 @@ def build():
 def _lib(precision: str):
     if precision in _LIBS:
         return _LIBS[precision]
-    path = os.path.join(_HERE, f"liboracle_{precision}.so")
+    # ORACLE_LIB_DIR: a directory of deliberately broken builds, used by the
+    # test that checks that the oracle's pinning tests catch planted mistakes
+    path = os.path.join(os.environ.get("ORACLE_LIB_DIR", _HERE), f"liboracle_{precision}.so")
     if not os.path.exists(path):
         build()
     lib = ctypes.CDLL(path)
@@ -88,6 +92,8 @@ def _lib(precision: str):
     lib.or_weiler_probe.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
                                     ctypes.c_int32, P(OrStats)]
     lib.or_weiler_probe.restype = ctypes.c_int32
+    lib.or_force_probe.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double)]
+    lib.or_force_probe.restype = ctypes.c_int32
     lib.or_rigid_frame.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double),
                                    P(ctypes.c_double)]
     lib.or_rigid_frame.restype = ctypes.c_int32
@@ -174,6 +180,14 @@ class OracleSim:
                                    _ptr(R, ctypes.c_double)) != 0:
             raise ValueError(f"no rigid body {body}")
         return c0, cm, R.reshape(3, 3)
+
+    def force_probe(self, which: int) -> np.ndarray:
+        """Accelerations of one force term on the current state (n x 3):
+        PROBE_VISCOSITY, PROBE_SURFACE_TENSION or PROBE_ELASTIC."""
+        out = np.zeros(self.n * 3, dtype=np.float64)
+        if self.lib.or_force_probe(self.h, int(which), _ptr(out, ctypes.c_double)) != 0:
+            raise ValueError(f"force probe {which} failed")
+        return out.reshape(self.n, 3)
 
     def weiler_probe(self):
         nf = int(np.count_nonzero(self.kind == 0))

@@ -1611,3 +1611,22 @@ int32_t or_weiler_probe(or_sim *s, double *mat, double *b, double *x, int32_t nm
     free(fidx);
     return 0;
 }
+
+/* Force probe for tests (see sph_oracle.h): one term on the current state. */
+int32_t or_force_probe(or_sim *s, int32_t which, double *out)
+{
+    if (which < 0 || which > 2) return -1;
+    if (which == 2 && s->ne > 0 && s->ns == 0) return -1;
+    or_search(s);
+    boundary_volume(s, 1);
+    if (s->nf > 0) densities(s);
+    for (int i = 0; i < 3 * s->n; i++) s->a[i] = 0;
+    if (which == 0) viscosity_explicit(s);
+    else if (which == 1) surface_tension(s);
+    else if (s->ne > 0) {
+        elastic_rot_stress(s);
+        elastic_forces(s);
+    }
+    for (int i = 0; i < 3 * s->n; i++) out[i] = (double)s->a[i];
+    return 0;
+}

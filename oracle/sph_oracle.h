@@ -119,6 +119,13 @@ void or_kernel_grad(const or_params *p, const double r[3], double out[3]);
 void or_polar(const double a[9], double r[9]);
 double or_viscosity_model(const or_params *p, double strain_rate_norm);
 int32_t or_weiler_probe(or_sim *s, double *mat, double *b, double *x, int32_t nmax, or_stats *st);
+/* One force term on the current state, for tests: search, dynamic boundary
+ * volumes and densities as a step forms them, then the accelerations of that
+ * term alone (out: n x 3, all other terms and gravity left out).
+ * which: 0 ViscosityModel (fluid term, and the boundary term with its
+ * reaction on dynamic solids when boundary_viscosity != 0), 1 SurfaceTension,
+ * 2 Elasticity (rotations, stresses, forces; needs or_init). */
+int32_t or_force_probe(or_sim *s, int32_t which, double *out);
 
 #ifdef __cplusplus
 }

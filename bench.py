@@ -10,14 +10,24 @@ Weiler implicit solver, 25% of scenes with an elastic block, dt 5e-4.
             particle advanced one step; boundary particles are not counted)
   plain     the same scenes stepped one at a time, one context each, over the
             same step indices as the batched form (same warm-up, same count)
+  control   one scene pooled against the same scene in a context (about 1)
   latency   p50/p99 scene-step latency: the tick time for the batched form,
             the per-scene step time for the plain form
-  roofline  the kernel with the largest share of a profiled tick: algorithmic
-            bytes per launch (per-kernel table below, from the logged per-scene
-            counts and iteration numbers) over its mean launch duration, from
-            HIP events around every 8th launch of it in the timed region
+  roofline  the kernel with the largest share of a probe tick (on a separate
+            pool): algorithmic bytes per launch (sweep table below, from the
+            logged per-scene counts and iteration numbers) over its mean
+            launch duration, from HIP events around every 8th launch of it in
+            the timed region; also a whole CG iteration and the whole tick
+  serving   continuous batching of the Poisson request stream (1k-500k
+            particles, 50-500 steps) at 5, 20 and 80 requests/s
   cpu       the oracle (FP32, the reference's arithmetic, OpenMP over particles)
-            on the host cores, on a bounded sample of the same scenes
+            on the host cores, on a bounded sample of the same scenes, and the
+            oracle as the checker: neighbor lists and fields of two scenes,
+            the dam break's density-error drift against the FP64 oracle
+
+The timed ticks are bracketed by roctxProfilerResume/Pause, so
+`rocprofv3 --selected-regions --kernel-trace --stats -- python bench.py ...`
+profiles exactly them.
 
     python bench.py [--steps K] [--warmup W] [--workload mixed_batch|dam_break|nonnewton_dam|elastic_drop]
 """
@@ -47,58 +57,113 @@ def log(*a):
 
 
 # ---------------------------------------------------------------------------
-# algorithmic bytes per launch, per scene (4-byte words; per-particle inputs
-# counted once, one word per neighbor-list entry walked).  N all particles,
-# Nf fluid, E = sum of neighbor counts, Ef = sum over fluid particles.
+# Algorithmic bytes: 4 bytes per word; per-particle inputs counted once for all
+# N particles a sweep touches, plus the words it writes, plus one word per
+# neighbor-list entry it walks (re-reads of neighbor data are free).  N all
+# particles of a scene, Nf fluid, Ne elastic, Nd dynamic solids (elastic and
+# rigid), Ndyn dynamic (fluid + Nd); E the list entries of the step's search
+# (fluid and dynamic-solid lists), Ef those of fluid particles, Ed those of
+# dynamic solids (also taken for the elastic rest lists).
 # ---------------------------------------------------------------------------
-def alg_bytes(kernel, N, Nf, Ne, E, Ef, warm=0):
-    w = {
-        "build_lists": 4 * N + E,               # x in; lists + counts out
-        "density_alpha": 8 * N + 4 * Nf + Ef,   # x,V,v,kind in; rho, alpha, rho-dot out
-        "div_update": 5 * N + 9 * Nf + Ef,      # x,V,kind; rho-dot, alpha; v in/out
-        "div_source": 8 * N + 2 * Nf + Ef,      # x,V,v,kind; rho-dot out
-        "press_update": 5 * N + 9 * Nf + Ef,
-        "press_source": 8 * N + 3 * Nf + Ef,    # + rho in
-        "nonpressure": 14 * N + 2 * Nf + Ef,    # x,V,v,kind, rho_j; acc and v' out
-        # the CG's 3-vectors are 3 words (u, w, p, s, x, r, e; M is 6)
-        "cg_matvec": 4 * N + 8 * Nf + 2 * Ef,   # x, kind; u, rho, count in; w out; list + pair scalar
-        "cg_update": N + 39 * Nf,               # kind; u, w, p, s, x, r, M in; p, s, x, r, u out
-        "cg_init": (13 + 3 * warm) * N + (10 + 3 * warm) * Nf + 2 * Ef,  # x,V,v,mu,rho,kind (+e) in,
-                                                # x out; count in, M, r0 (+Ae) out; list in, pair scalar out
-        "cg_guess": N + (21 + 6 * warm) * Nf,   # kind; v, r0, M (+e, Ae) in; x, r, u out
-        "strain": 8 * N + 3 * Nf + Ef,
-        "advect": 12 * N,
-        "clear_acc": 5 * N,
-        "elastic_rot_stress": 23 * Ne,
-        "elastic_force": 26 * Ne,
-    }.get(kernel)
-    return None if w is None else 4 * w
+def sweep_words(sweep, N, Nf, Ne, Nd, E, Ef, Ed):
+    return {
+        "search": 4 * N + E,                 # x in; cell offsets and lists out
+        "density_alpha": 5 * N + 2 * Nf + Ef,
+        "density_change": 7 * N + Nf + Ef,   # + its FP64 reduction
+        "div_update": 5 * N + 8 * Nf + Ef,
+        "nonpressure": 7 * N + 5 * Nf + Ef,  # viscosity, surface tension, advect of v
+        "density_adv": 7 * N + 2 * Nf + Ef,  # + its FP64 reduction
+        "press_update": 5 * N + 8 * Nf + Ef,
+        "advect": 12 * (Nf + Nd),
+        "boundary_volume": 4 * N + Nd + Ed,
+        "elastic_rot_stress": 8 * Ne + 15 * Ne + Ed,
+        "elastic_force": 23 * Ne + 3 * Ne + Ed,
+        # one Weiler CG iteration: w = A u over the lists (x, u, rho, kind of
+        # fluid particles in, w out: 11 Nf + Ef) and the vector updates with
+        # the block preconditioner (37 Nf) -- 9 Nf + 3 Nf + Ef + 36 Nf in all
+        "cg_matvec": 11 * Nf + Ef,
+        "cg_update": 37 * Nf,
+        "strain": 8 * N + Nf + Ef,
+    }[sweep]
 
 
-# iteration kernels run once per solver iteration of each active scene
-ITER_KERNEL = {"div_update": "iterations_v", "div_source": "iterations_v", "press_update": "iterations_p",
-               "cg_update": "iterations_cg"}
+def scene_counts(sc, st):
+    nd = sc.n_elastic + sc.n_rigid
+    E, Ef = st["num_pairs"], st["num_pairs_fluid"]
+    return dict(N=sc.n, Nf=sc.n_fluid, Ne=sc.n_elastic, Nd=nd, E=E, Ef=Ef, Ed=E - Ef)
 
 
-def scene_algorithmic_bytes(kernel, sc, st):
-    b = alg_bytes(kernel, sc.n, sc.n_fluid, sc.n_elastic, st["num_pairs"], st["num_pairs_fluid"],
-                  int(sc.params.warm_start) & 1)
-    if b is None:
-        return None
+def _weiler(sc):
     p = sc.params
-    weiler = sc.n_fluid > 0 and p.use_viscosity and p.viscosity_method == 1
-    if kernel.startswith("cg_") and not weiler:
-        return 0
-    if kernel == "strain" and not (weiler or p.use_non_newtonian):
-        return 0
-    if kernel in ITER_KERNEL:
-        return b * st[ITER_KERNEL[kernel]]
-    if kernel == "press_source":  # one launch before the loop plus one per iteration
-        return b * (1 + st["iterations_p"])
-    if kernel == "cg_matvec":  # one launch at the start of the solve plus one per iteration
-        return b * (1 + st["iterations_cg"])
+    return sc.n_fluid > 0 and p.use_viscosity and p.viscosity_method == 1
+
+
+def kernel_algorithmic_bytes(kernel, sc, st):
+    """Algorithmic bytes of `kernel`'s launches for scene `sc` in one step with
+    statistics `st`; None for kernels without a table entry."""
+    c = scene_counts(sc, st)
+    p = sc.params
+    Iv, Ip, Icg = st["iterations_v"], st["iterations_p"], st["iterations_cg"]
+    fluid = sc.n_fluid > 0
+
+    def w(name):
+        return 4 * sweep_words(name, **c)
+
+    if kernel == "build_lists":
+        return w("search")
+    if kernel == "density_alpha":  # density and alpha, and the first density change (fused)
+        return (w("density_alpha") + (w("density_change") if p.enable_divergence_solver else 0)) if fluid else 0
     if kernel == "div_source":
-        return b * st["iterations_v"]
+        return Iv * w("density_change") if fluid else 0
+    if kernel == "div_update":
+        return Iv * w("div_update") if fluid else 0
+    if kernel == "nonpressure":
+        return w("nonpressure") if fluid else 0
+    if kernel == "press_source":
+        return (1 + Ip) * w("density_adv") if fluid else 0
+    if kernel == "press_update":
+        return Ip * w("press_update") if fluid else 0
+    if kernel == "advect":
+        return w("advect")
+    if kernel == "bvol":
+        return w("boundary_volume") if c["Nd"] else 0
+    if kernel in ("elastic_rot_stress", "elastic_force"):
+        return w(kernel) if c["Ne"] else 0
+    if kernel == "strain":
+        return w("strain") if (_weiler(sc) or p.use_non_newtonian) else 0
+    if kernel == "cg_matvec":  # the solve's first product, then one per iteration
+        return (1 + Icg) * w("cg_matvec") if _weiler(sc) else 0
+    if kernel == "cg_update":
+        return Icg * w("cg_update") if _weiler(sc) else 0
+    return None
+
+
+def step_algorithmic_bytes(sc, st):
+    """The whole step: search + density/alpha + (1 + Iv) density changes + Iv
+    divergence updates + non-pressure + (1 + Ip) predicted densities + Ip
+    pressure updates + advect, with the boundary-volume, elastic, strain and
+    Weiler terms (Icg iterations) where the scene has them."""
+    c = scene_counts(sc, st)
+    p = sc.params
+    Iv, Ip, Icg = st["iterations_v"], st["iterations_p"], st["iterations_cg"]
+
+    def w(name):
+        return 4 * sweep_words(name, **c)
+
+    b = w("search") + w("advect")
+    if sc.n_fluid:
+        b += w("density_alpha") + w("nonpressure")
+        if p.enable_divergence_solver:
+            b += (1 + Iv) * w("density_change") + Iv * w("div_update")
+        b += (1 + Ip) * w("density_adv") + Ip * w("press_update")
+        if _weiler(sc) or p.use_non_newtonian:
+            b += w("strain")
+        if _weiler(sc):
+            b += Icg * (w("cg_matvec") + w("cg_update"))
+    if c["Nd"]:
+        b += w("boundary_volume")
+    if c["Ne"]:
+        b += w("elastic_rot_stress") + w("elastic_force")
     return b
 
 
@@ -124,7 +189,38 @@ def load_workload(name):
     return [sc], f"{name}: {sc.n} particles ({sc.n_dynamic} dynamic), dt {sc.params.dt}"
 
 
-EVENT_STRIDE = 8  # the dominant kernel is timed on every 8th launch in the timed region
+EVENT_STRIDE = 8  # the timed kernels carry HIP events on every 8th launch in the timed region
+
+
+class TimedRegion:
+    """Brackets the timed ticks with roctxProfilerResume / roctxProfilerPause,
+    so that `rocprofv3 --selected-regions` records exactly the launches the
+    line reports on (without a profiler the calls do nothing)."""
+
+    def __init__(self):
+        import ctypes
+
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            for fn in (self.lib.roctxProfilerResume, self.lib.roctxProfilerPause):
+                fn.argtypes = [ctypes.c_uint64]
+                fn.restype = ctypes.c_int
+
+    def __enter__(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+        return False
 
 
 def _admit_all(scs, block):
@@ -137,56 +233,62 @@ def _admit_all(scs, block):
     return pool, slots
 
 
-def run_batched(scs, steps, warmup, profile=True, block=0, tune=True, kernel=None):
-    """The batched form.  Launch shapes are tuned on a separate pool holding the
-    same scenes (the tuner advances its scenes), so the timed pool starts from
-    the same state whether tuned or not."""
+def run_batched(scs, steps, warmup, profile=True, block=0, tune=True, kernel=None, region=None):
+    """The batched form.  Launch shapes are tuned, and one tick with every
+    kernel timed picks the dominant kernel, on a separate pool holding the same
+    scenes; the timed pool then runs `warmup` ticks and the `steps` timed ticks,
+    so it times the same step indices as the plain form.  In the timed region
+    the dominant kernel and the CG kernels carry HIP events on every
+    EVENT_STRIDE-th launch."""
     from nonnewton_amd import autotune
     shapes = {}
+    breakdown = {}
+    tp, tslots = _admit_all(scs, block)
+    for k in range(warmup):
+        tp.tick()
     if tune:
-        tp, tslots = _admit_all(scs, block)
-        for k in range(warmup):
-            tp.tick()
         shapes = autotune.tune_pool(tp, tslots, [s.n_fluid for s in scs], use_cache=False)
-        tp.close()
         log(f"[bench] launch shapes: {shapes}")
+    if profile:
+        tp.profile(True)
+        tp.tick()
+        breakdown = tp.kernel_times()
+        tp.profile(False)
+    tp.close()
     t0 = time.time()
     pool, slots = _admit_all(scs, block)
     autotune.apply(pool, shapes)
     log(f"[bench] admitted {len(slots)} scenes in {time.time() - t0:.2f} s")
     for k in range(warmup):
         pool.tick()
-    breakdown = {}
     dominant = None
+    timed_kernels = []
     if profile:
-        # one more untimed tick with every kernel timed picks the dominant
-        # kernel; the timed region then records events around every
-        # EVENT_STRIDE-th launch of that kernel only
-        pool.profile(True)
-        pool.tick()
-        breakdown = pool.kernel_times()
         dominant = kernel or max(breakdown.items(), key=lambda kv: kv[1][0])[0]
-        pool.profile(True, kernels=[dominant], stride=EVENT_STRIDE)
+        timed_kernels = [dominant] + [k for k in ("cg_matvec", "cg_update") if k != dominant]
+        pool.profile(True, kernels=timed_kernels, stride=EVENT_STRIDE)
     lat = []
     raw = []
-    t_start = time.perf_counter()
-    for k in range(steps):
-        t = time.perf_counter()
-        pool.tick()
-        lat.append(time.perf_counter() - t)
-        raw.append(pool.stats_array(slots))
-    elapsed = time.perf_counter() - t_start
+    with (region or TimedRegion()):
+        t_start = time.perf_counter()
+        for k in range(steps):
+            t = time.perf_counter()
+            pool.tick()
+            lat.append(time.perf_counter() - t)
+            raw.append(pool.stats_array(slots))
+        elapsed = time.perf_counter() - t_start
     names = raw[0].dtype.names if raw else ()
     per_step_stats = [[{f: a[f][k].item() for f in names} for k in range(len(slots))] for a in raw]
     times = {}
     launches = {}
     if profile:
-        ms, timed = pool.kernel_times()[dominant]
+        kt = pool.kernel_times()
         launches = pool.kernel_launches()  # every kernel's launches in the timed region
-        times = {dominant: (ms, timed, launches[dominant])}
+        times = {k: (kt[k][0], kt[k][1], launches[k]) for k in timed_kernels if launches.get(k)}
     pool.profile(False)
     pool.close()
-    return elapsed, lat, per_step_stats, times, breakdown, shapes, launches
+    return dict(elapsed=elapsed, lat=lat, stats=per_step_stats, times=times, dominant=dominant,
+                breakdown=breakdown, shapes=shapes, launches=launches)
 
 
 def run_plain(scs, steps, warmup, block=0, shapes=None):
@@ -219,6 +321,29 @@ def run_plain(scs, steps, warmup, block=0, shapes=None):
     return elapsed, lat, np.array(its, dtype=np.float64)
 
 
+def one_scene_control(sc, steps, warmup, shapes):
+    """One scene in a pool against the same scene in a context, same steps: the
+    two paths run the same kernels, so the ratio is about 1 (the pool pays its
+    tables and scene bookkeeping), and the batched form's gain over the plain
+    one comes from stepping many scenes per launch, not from a faster path."""
+    from nonnewton_amd import autotune
+
+    pool, slots = _admit_all([sc], 0)
+    autotune.apply(pool, shapes)
+    for _ in range(warmup):
+        pool.tick()
+    t = time.perf_counter()
+    for _ in range(steps):
+        pool.tick()
+    pooled = time.perf_counter() - t
+    pool.close()
+    plain, _, _ = run_plain([sc], steps, warmup, shapes=shapes)
+    return {"scene": sc.name, "dynamic_particles": sc.n_dynamic,
+            "pooled_particle_steps_per_s": round(sc.n_dynamic * steps / pooled, 1),
+            "plain_particle_steps_per_s": round(sc.n_dynamic * steps / plain, 1),
+            "pooled_over_plain": round(plain / pooled, 3)}
+
+
 def run_serving(count, rate):
     """Continuous batching over a Poisson request stream (sizes log-uniform 1k-500k,
     50-500 steps, seed 4), 200 GB budget."""
@@ -230,46 +355,112 @@ def run_serving(count, rate):
     loop.close()
     s = rep.summary()
     s["rate_per_s"] = rate
+    s["failed"] = int(sum(r.failed for r in rep.records))
     s["dynamic_particles_requested"] = int(sum(q.scene.n_dynamic for q in reqs))
     s["steps_requested"] = int(sum(q.steps for q in reqs))
     return s
 
 
-def cpu_baseline(scs, budget_s):
+def accuracy_checks(scs):
+    """Against the oracle: (1) two scenes of the workload stepped 3 times on the
+    GPU and in the FP32 oracle -- neighbor lists equal as sets and in order,
+    fields bit for bit; (2) the dam break (13,824 fluid + 8,706 wall particles,
+    dt 1e-3) 100 steps on the GPU against the FP64 oracle: the drift of the
+    average density error (largest difference over the steps, % of rho0) and
+    the largest error either run reached."""
+    from oracle import oracle as O
+
+    out = {}
+    picks = sorted(range(len(scs)), key=lambda k: scs[k].n)[:2]
+    same = True
+    for k in picks:
+        sc = scs[k]
+        c = L.Context(sc.params, sc.n)
+        c.upload(sc.x, sc.kind, sc.v)
+        c.init()
+        o = O.OracleSim.from_scene(sc, precision="f32")
+        o.init()
+        for _ in range(3):
+            c.step()
+            o.step()
+        nb_g, c_g = c.neighbors()
+        nb_o, c_o = o.neighbors()
+        same &= bool(np.array_equal(c_g, c_o) and np.array_equal(nb_g, nb_o))
+        for fg, fo in ((L.FIELD_X, O.F_X), (L.FIELD_V, O.F_V), (L.FIELD_DENSITY, O.F_DENSITY)):
+            b = o.get(fo)
+            a = c.download(fg).astype(np.float64).reshape(b.shape)
+            if sc.meta.get("model", 0) == 0:
+                same &= bool(np.array_equal(a, b))
+            else:  # powf of the non-Newtonian models: last-bit differences between libraries
+                same &= bool(np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-12))
+        c.close()
+    out["neighbor_sets_and_fields_vs_fp32_oracle"] = {
+        "scenes": [scs[k].name for k in picks], "steps": 3, "equal": same}
+    db = S.dam_break()
+    c = L.Context(db.params, db.n)
+    c.upload(db.x, db.kind, db.v)
+    c.init()
+    o = O.OracleSim.from_scene(db, precision="f64")
+    o.init()
+    rho0 = db.params.density0
+    drift = 0.0
+    worst = 0.0
+    for _ in range(100):
+        sg = c.step()
+        so = o.step()
+        drift = max(drift, abs(sg["avg_density_err_p"] - so["avg_density_err_p"]))
+        worst = max(worst, sg["avg_density_err_p"], so["avg_density_err_p"])
+    c.close()
+    out["dam_break_100_steps_vs_fp64"] = {
+        "avg_density_error_drift_pct": float(f"{100 * drift / rho0:.3g}"),
+        "max_avg_density_error_pct": float(f"{100 * worst / rho0:.3g}")}
+    return out
+
+
+def cpu_baseline(scs, budget_s, warm_steps=10):
     """The oracle in FP32 without compensated sums (the reference's arithmetic)
-    on the host cores: one untimed step, then timed steps, scene after scene,
-    until about budget_s seconds of timed CPU work."""
+    on the host cores: `warm_steps` untimed steps per scene, then timed steps,
+    round robin over the scenes, until about budget_s seconds of timed CPU
+    work; the median scene-step time beside the aggregate rate."""
     from oracle import oracle as O
 
     cores = len(os.sched_getaffinity(0))
     O.set_threads(cores, "f32")
     done_dyn = 0
-    steps = 0
     spent = 0.0
     sims = []
     for s in scs:
         sim = O.OracleSim.from_scene(s, precision="f32", compensated=False)
         sim.init()
-        sim.step()
+        for _ in range(warm_steps):
+            sim.step()
         sims.append((s, sim))
     passes = 0
+    per_scene = [[] for _ in sims]
     while spent < budget_s:
         # round robin over every scene, so the sample has the workload's mix
-        for s, sim in sims:
+        for q, (s, sim) in enumerate(sims):
             t = time.perf_counter()
             sim.step()
-            spent += time.perf_counter() - t
+            dt = time.perf_counter() - t
+            spent += dt
+            per_scene[q].append(dt)
             done_dyn += s.n_dynamic
-            steps += 1
         passes += 1
+    steps = sum(len(v) for v in per_scene)
+    med = [float(np.median(v)) for v in per_scene]
     return {
         "value": done_dyn / spent,
         "unit": "particle-steps/s",
         "cores": int(O.num_threads("f32")),
         "kind": "port",
-        "sample": f"oracle FP32 (reference arithmetic, OpenMP over particles): {passes} round-robin pass(es) of one "
-                  f"timed step over all {len(sims)} scenes of the workload after one untimed step each; {steps} "
-                  f"scene-steps, {done_dyn} particle-steps, {spent:.1f} s of timed CPU work",
+        "median_scene_step_ms": round(1e3 * float(np.median(np.concatenate([np.asarray(v) for v in per_scene]))), 3),
+        "median_round_ms": round(1e3 * sum(med), 1),
+        "boundary_particles": int(sum(s.n_static for s, _ in sims)),
+        "sample": f"oracle FP32 (reference arithmetic, OpenMP over particles): {warm_steps} untimed steps per scene, "
+                  f"then {passes} round-robin pass(es) of one timed step over all {len(sims)} scenes of the workload; "
+                  f"{steps} scene-steps, {done_dyn} particle-steps, {spent:.1f} s of timed CPU work; "
+                  f"median_round_ms = sum over scenes of the median step time",
     }
 
 
@@ -360,6 +551,15 @@ def run_surface(n, reps, cpu_particles):
     print(json.dumps(out), flush=True)
 
 
+def roofline_entry(name, kernel_bytes, avg_s, launches, elapsed, workload):
+    achieved = kernel_bytes / launches / avg_s / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, workload),
+            "launches": launches, "avg_launch_us": round(1e6 * avg_s, 2),
+            "bytes_per_launch": round(kernel_bytes / launches),
+            "share_of_timed_region": round(avg_s * launches / elapsed, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -368,12 +568,13 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=0, help="timed steps of the plain form (0 = --steps)")
     ap.add_argument("--workload", default="mixed_batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline and oracle checks)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-control", action="store_true", help="skip the one-scene pooled/plain control")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
-    ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=64, help="requests per arrival rate (0 = no serving)")
+    ap.add_argument("--serve-rates", default="5,20,80", help="Poisson arrival rates, requests/s")
     ap.add_argument("--roofline-kernel", default=None, metavar="KERNEL",
                     help="report the roofline of KERNEL instead of the one that dominates the probe tick")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -401,11 +602,16 @@ def main():
     n_all = sum(s.n for s in scs)
     log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
 
-    elapsed, lat, stats, ktimes, breakdown, shapes, launch_counts = run_batched(
-        scs, args.steps, args.warmup, block=args.block, tune=not args.no_tune, kernel=args.roofline_kernel)
+    b = run_batched(scs, args.steps, args.warmup, block=args.block, tune=not args.no_tune,
+                    kernel=args.roofline_kernel)
+    elapsed, lat, stats, shapes = b["elapsed"], b["lat"], b["stats"], b["shapes"]
     value = n_dyn * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] batched: {ms_per_step:.3f} ms/tick, {value:.4g} particle-steps/s")
+    if b["breakdown"]:
+        log("[bench] kernel times of the probe tick (ms, launches): " +
+            ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in sorted(b["breakdown"].items(), key=lambda kv: -kv[1][0])
+                      if v[1]))
 
     plain_value = None
     plain_lat = []
@@ -418,44 +624,42 @@ def main():
         log(f"[bench] plain: {1e3 * pe / plain_steps:.3f} ms per round of {len(scs)} scenes, "
             f"{plain_value:.4g} particle-steps/s")
 
-    # roofline of the dominant kernel
+    # roofline: the dominant kernel (the line's figure), a whole CG iteration
+    # (matvec + update) and the whole tick, from algorithmic bytes over the
+    # timed ticks and the event-timed mean launch durations
     roof = None
-    if ktimes:
-        name, (ms, timed, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
-        total_bytes = 0.0
-        ok = True
-        for step_stats in stats:
-            for sc, st in zip(scs, step_stats):
-                b = scene_algorithmic_bytes(name, sc, st)
-                if b is None:
-                    ok = False
-                    break
-                total_bytes += b
-        if ok and launches and timed:
-            avg_s = ms * 1e-3 / timed          # mean duration of the timed (every EVENT_STRIDE-th) launches
-            per_launch = total_bytes / launches  # algorithmic bytes per launch over all launches
-            achieved = per_launch / avg_s / 1e9
-            roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(name, args.workload),
-                    "launches": launches, "timed_launches": timed, "avg_launch_us": round(1e6 * avg_s, 2),
-                    "bytes_per_launch": round(per_launch),
-                    "share_of_timed_region": round(avg_s * launches / elapsed, 3)}
-        # algorithmic bytes per launch of the other CG kernels over the same ticks
-        # (the rocprofv3 traffic summaries compare their counters with these)
-        per_kernel = {}
-        for k in ("cg_matvec", "cg_update"):
-            n_l = launch_counts.get(k, 0)
-            tb = sum(scene_algorithmic_bytes(k, sc, st) or 0 for ss in stats for sc, st in zip(scs, ss))
-            if n_l:
-                per_kernel[k] = round(tb / n_l)
-        if roof is not None:
-            roof["alg_bytes_per_launch"] = per_kernel
-        log("[bench] kernel times of one tick (ms, launches): " +
-            ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0])
-                      if v[1]))
+    extra = {}
+    times = b["times"]
+    if times:
+        kb = {k: sum(kernel_algorithmic_bytes(k, sc, st) or 0 for ss in stats for sc, st in zip(scs, ss))
+              for k in times}
+        avg = {k: (ms * 1e-3 / timed) for k, (ms, timed, _) in times.items() if timed}
+        name = b["dominant"]
+        if name in avg and kernel_algorithmic_bytes(name, scs[0], stats[0][0]) is not None:
+            roof = roofline_entry(name, kb[name], avg[name], times[name][2], elapsed, args.workload)
+            roof["timed_launches"] = times[name][1]
+        if "cg_matvec" in avg and "cg_update" in avg:
+            t_it = avg["cg_matvec"] * times["cg_matvec"][2] + avg["cg_update"] * times["cg_update"][2]
+            by = kb["cg_matvec"] + kb["cg_update"]
+            extra["cg_iteration"] = {"achieved": round(by / t_it / 1e9, 1),
+                                     "frac": round(by / t_it / 1e9 / HBM_PEAK_GBS, 4),
+                                     "bytes_per_tick": round(by / args.steps),
+                                     "ms_per_tick": round(1e3 * t_it / args.steps, 3)}
+            if name != "cg_update" and "cg_update" in avg:
+                extra["cg_update"] = roofline_entry("cg_update", kb["cg_update"], avg["cg_update"],
+                                                    times["cg_update"][2], elapsed, args.workload)
+    tick_bytes = sum(step_algorithmic_bytes(sc, st) for ss in stats for sc, st in zip(scs, ss))
+    extra["tick"] = {"achieved": round(tick_bytes / elapsed / 1e9, 1),
+                     "frac": round(tick_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes_per_tick": round(tick_bytes / args.steps)}
+    if roof is not None:
+        roof.update(extra)
+    log(f"[bench] roofline: {json.dumps(roof)}")
 
     it = np.array([[st["iterations_v"], st["iterations_p"], st["iterations_cg"]] for ss in stats for st in ss])
+    rho0 = np.array([sc.params.density0 for sc in scs])
+    err_p = np.array([[st["avg_density_err_p"] for st in ss] for ss in stats]) / rho0[None, :]
+    nan = any(st["status"] & 8 for ss in stats for st in ss)
     cfg = {
         "workload": workload,
         "scenes": len(scs),
@@ -465,6 +669,11 @@ def main():
         "p50_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 50)), 3),
         "p99_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 99)), 3),
         "mean_iterations_v_p_cg": [round(float(x), 2) for x in it.mean(axis=0)],
+        "max_iterations_v_p_cg": [int(x) for x in it.max(axis=0)],
+        # the pressure solve's average density error (FP64 reduction), worst
+        # scene and tick of the timed region, in % of rho0
+        "max_avg_density_error_pct": float(f"{100 * float(err_p.max()):.3g}"),
+        "non_finite_state": bool(nan),
         "launch_shapes": shapes,
     }
     if plain_value is not None:
@@ -474,15 +683,26 @@ def main():
         cfg["p99_scene_step_ms_plain"] = round(1e3 * float(np.percentile(plain_lat, 99)), 3)
         cfg["plain_steps_timed"] = plain_steps
         cfg["mean_iterations_v_p_cg_plain"] = [round(float(x), 2) for x in plain_its.mean(axis=0)]
+    if not args.no_control and len(scs) > 1:
+        mid = sorted(scs, key=lambda s: s.n_dynamic)[len(scs) // 2]
+        cfg["one_scene_control"] = one_scene_control(mid, args.steps, args.warmup, shapes)
+        log(f"[bench] one-scene control: {cfg['one_scene_control']}")
 
     if args.serve_requests > 0 and args.workload == "mixed_batch":
-        cfg["serving"] = run_serving(args.serve_requests, args.serve_rate)
-        log(f"[bench] serving: {cfg['serving']}")
+        cfg["serving"] = []
+        for rate in [float(r) for r in args.serve_rates.split(",") if r]:
+            s = run_serving(args.serve_requests, rate)
+            cfg["serving"].append(s)
+            log(f"[bench] serving at {rate:g}/s: {s}")
 
     cpu = None
     if not args.no_cpu:
+        # the CPU leg: the oracle timed on the host cores (the baseline) and
+        # used as the checker of the GPU results (accuracy)
         cpu = cpu_baseline(scs, args.cpu_seconds)
         log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
+        cfg["accuracy"] = accuracy_checks(scs)
+        log(f"[bench] accuracy: {cfg['accuracy']}")
 
     out = {
         "metric": METRIC,

@@ -130,6 +130,9 @@ typedef struct nn_step_stats {
 #define NN_FIELD_X0 13          /* float[n][3] (ParticleData.x_0) */
 #define NN_FIELD_KIND 14        /* int32[n] */
 #define NN_FIELD_NUM_NEIGHBORS 15 /* int32[n] (NeighborhoodSearchSparse.num_neighbors) */
+#define NN_FIELD_VISC_DIFF 16   /* float[n][3]: x - v of the previous Weiler solve, the warm
+                                 * start's direction (Viscosity_Weiler2018.m_vDiff, :1486-1487);
+                                 * with x and v it is the state a run continues from */
 
 typedef struct nn_ctx nn_ctx;
 typedef struct nn_pool nn_pool;
@@ -158,7 +161,8 @@ int32_t nn_step(nn_ctx *ctx, int32_t nsteps, nn_step_stats *out);
 int32_t nn_search(nn_ctx *ctx);
 
 /* Replaces meta.pd.<field>.to_numpy() / .from_numpy() (:740-741, :940, :1548).
- * bytes must equal n * width * 4. */
+ * bytes must equal n * width * 4.  nn_upload_field takes NN_FIELD_X, NN_FIELD_V
+ * and NN_FIELD_VISC_DIFF. */
 int32_t nn_download(nn_ctx *ctx, int32_t field, void *host, size_t bytes);
 int32_t nn_upload_field(nn_ctx *ctx, int32_t field, const void *host, size_t bytes);
 

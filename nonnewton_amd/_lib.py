@@ -63,8 +63,10 @@ FIELD_REST_VOLUME = 12
 FIELD_X0 = 13
 FIELD_KIND = 14
 FIELD_NUM_NEIGHBORS = 15
+FIELD_VISC_DIFF = 16
 
-FIELD_WIDTH = {FIELD_X: 3, FIELD_V: 3, FIELD_ACCELERATION: 3, FIELD_X0: 3, FIELD_ROTATION: 9, FIELD_STRESS: 6}
+FIELD_WIDTH = {FIELD_X: 3, FIELD_V: 3, FIELD_ACCELERATION: 3, FIELD_X0: 3, FIELD_ROTATION: 9, FIELD_STRESS: 6,
+               FIELD_VISC_DIFF: 3}
 INT_FIELDS = {FIELD_KIND, FIELD_NUM_NEIGHBORS}
 
 MAX_NEIGHBORS = 60

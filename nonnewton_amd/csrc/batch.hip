@@ -1096,6 +1096,7 @@ static int field_width(int field) {
     case NN_FIELD_V:
     case NN_FIELD_ACCELERATION:
     case NN_FIELD_X0:
+    case NN_FIELD_VISC_DIFF:
       return 3;
     case NN_FIELD_ROTATION:
       return 9;
@@ -1157,8 +1158,10 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
     case NN_FIELD_X:
     case NN_FIELD_V:
     case NN_FIELD_ACCELERATION:
+    case NN_FIELD_VISC_DIFF:
     case NN_FIELD_VOLUME: {
-      const float4* p = field == NN_FIELD_V ? d.vel : field == NN_FIELD_ACCELERATION ? d.acc : d.pos;
+      const float4* p = field == NN_FIELD_V ? d.vel : field == NN_FIELD_ACCELERATION ? d.acc
+                        : field == NN_FIELD_VISC_DIFF ? d.vdiff : d.pos;
       HIPCHK(slot_f4(p, f4));
       HIPCHK(hipStreamSynchronize(stream));
       for (int t = 0; t < n; t++) {
@@ -1259,13 +1262,13 @@ int Batch::download(int k, int field, void* host, size_t bytes) {
 int Batch::upload_field(int k, int field, const void* host, size_t bytes) {
   SceneHost& s = *scenes[k];
   const int n = s.n, pb = s.p_begin;
-  if ((field != NN_FIELD_X && field != NN_FIELD_V) || bytes != (size_t)n * 12) {
-    set_error("only x and v (float[n][3]) can be uploaded");
+  if ((field != NN_FIELD_X && field != NN_FIELD_V && field != NN_FIELD_VISC_DIFF) || bytes != (size_t)n * 12) {
+    set_error("only x, v and the Weiler warm-start direction (float[n][3]) can be uploaded");
     return NN_E_ARG;
   }
   std::vector<int> info(n);
   std::vector<float4> f4(n);
-  float4* p = field == NN_FIELD_X ? d.pos : d.vel;
+  float4* p = field == NN_FIELD_X ? d.pos : field == NN_FIELD_V ? d.vel : d.vdiff;
   HIPCHK(hipMemcpyAsync(info.data(), d.info + pb, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(f4.data(), p + pb, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));

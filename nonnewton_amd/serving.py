@@ -35,6 +35,7 @@ class RequestRecord:
     slot: Optional[int] = None
     done_steps: int = 0
     failed: bool = False
+    error: str = ""  # why it failed: rejected at admission or evicted by a step's status
     result: Optional[Dict[str, np.ndarray]] = None
 
 
@@ -75,6 +76,7 @@ class ServingLoop:
 
     def __init__(self, budget_bytes: int = 200 << 30, block: int = 0, keep_results: bool = False):
         self.pool = _lib.Pool(budget_bytes)
+        self.budget = int(budget_bytes)
         if block:
             self.pool.set_block(block)
         self.keep_results = keep_results
@@ -95,6 +97,17 @@ class ServingLoop:
         pending: List[RequestRecord] = list(order)
         state = {"stop": False, "freed": 0}
 
+        def reject_locked(rec, why):  # with self._cond held
+            pending.remove(rec)
+            rec.failed = True
+            rec.error = why
+            rec.finished = time.perf_counter() - t0
+            self._cond.notify_all()
+
+        def reject(rec, why):
+            with self._cond:
+                reject_locked(rec, why)
+
         def admitter():
             while True:
                 with self._cond:
@@ -109,11 +122,25 @@ class ServingLoop:
                         continue
                     freed_seen = state["freed"]
                 sc = scenes[nxt.rid]
-                slot = self.pool.admit(sc.params, sc.x, sc.kind, sc.v)
+                # a scene is held twice while it joins the batch: one that
+                # needs more than the whole budget that way can never be admitted
+                fp = self.pool.footprint(sc.params, sc.n, sc.n_elastic)
+                if fp < 0 or 2 * fp > self.budget:
+                    reject(nxt, f"needs {2 * fp} bytes to join, over the budget of {self.budget}")
+                    continue
+                try:
+                    slot = self.pool.admit(sc.params, sc.x, sc.kind, sc.v)
+                except _lib.NNError as e:  # out of grid, cell/neighbor overflow, bad kinds ...
+                    reject(nxt, str(e))
+                    continue
                 with self._cond:
                     if slot is None:  # over budget: wait for a retirement
                         while state["freed"] == freed_seen and not state["stop"]:
+                            if not resident:  # nothing will retire: the request cannot fit now
+                                break
                             self._cond.wait(timeout=0.05)
+                        if not resident and state["freed"] == freed_seen:
+                            reject_locked(nxt, "over the budget with nothing resident")
                         continue
                     pending.pop(0)
                     nxt.slot = slot
@@ -121,7 +148,15 @@ class ServingLoop:
                     resident[slot] = nxt
                     self._cond.notify_all()
 
-        th = threading.Thread(target=admitter, daemon=True)
+        def admitter_guarded():
+            try:
+                admitter()
+            except BaseException as e:  # keep the main loop from waiting on a dead thread
+                with self._cond:
+                    state["admit_error"] = e
+                    self._cond.notify_all()
+
+        th = threading.Thread(target=admitter_guarded, daemon=True)
         th.start()
         tick_ms: List[float] = []
         ticks = 0
@@ -131,10 +166,14 @@ class ServingLoop:
         try:
             while True:
                 with self._cond:
+                    if "admit_error" in state:
+                        raise RuntimeError("admission thread failed") from state["admit_error"]
                     active = {s: r for s, r in resident.items() if r.finished is None}
                     if not active and not pending:
                         break
                     if not active:
+                        if not th.is_alive() and pending:
+                            raise RuntimeError("admission thread ended with requests pending")
                         self._cond.wait(timeout=0.01)
                         continue
                 t = time.perf_counter()

@@ -26,7 +26,8 @@ MAX_NB = 60
 
 F_X, F_V, F_ACC, F_VOLUME, F_DENSITY, F_DFSPH_FACTOR, F_DENSITY_ADV = 0, 1, 2, 3, 4, 5, 6
 F_STRAIN_NORM, F_VISCOSITY, F_ROTATION, F_STRESS, F_F, F_REST_VOLUME, F_X0 = 7, 8, 9, 10, 11, 12, 13
-_WIDTH = {F_X: 3, F_V: 3, F_ACC: 3, F_X0: 3, F_ROTATION: 9, F_F: 9, F_STRESS: 6}
+F_VISC_DIFF = 16
+_WIDTH = {F_X: 3, F_V: 3, F_ACC: 3, F_X0: 3, F_ROTATION: 9, F_F: 9, F_STRESS: 6, F_VISC_DIFF: 3}
 
 PROBE_VISCOSITY, PROBE_SURFACE_TENSION, PROBE_ELASTIC = 0, 1, 2
 
@@ -162,7 +163,7 @@ class OracleSim:
     def set(self, field: int, values) -> None:
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n * 3)
         if self.lib.or_set(self.h, field, _ptr(a, ctypes.c_double)) != 0:
-            raise ValueError("only x and v can be set")
+            raise ValueError("only x, v and the Weiler warm-start direction can be set")
 
     def neighbors(self):
         nbr = np.zeros((self.n, MAX_NB), dtype=np.int32)

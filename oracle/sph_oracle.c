@@ -1454,6 +1454,7 @@ int32_t or_get(const or_sim *s, int32_t field, double *out)
     case OR_F_V: for (int i = 0; i < 3 * n; i++) out[i] = s->v[i]; break;
     case OR_F_ACC: for (int i = 0; i < 3 * n; i++) out[i] = s->a[i]; break;
     case OR_F_X0: for (int i = 0; i < 3 * n; i++) out[i] = s->x0[i]; break;
+    case OR_F_VISC_DIFF: for (int i = 0; i < 3 * n; i++) out[i] = s->wvd[i]; break;
     case OR_F_VOLUME: for (int i = 0; i < n; i++) out[i] = s->vol[i]; break;
     case OR_F_DENSITY: for (int i = 0; i < n; i++) out[i] = s->rho[i]; break;
     case OR_F_DFSPH_FACTOR: for (int i = 0; i < n; i++) out[i] = s->alpha[i]; break;
@@ -1494,10 +1495,11 @@ int32_t or_rigid_frame(const or_sim *s, int32_t b, double rest_cm[3], double cm[
     return 0;
 }
 
-/* overwrite positions or velocities (tests pose deformed states after init) */
+/* overwrite positions, velocities or the warm start's direction (tests pose
+ * deformed states after init, or continue a run from another's state) */
 int32_t or_set(or_sim *s, int32_t field, const double *in)
 {
-    real *dst = field == OR_F_X ? s->x : field == OR_F_V ? s->v : NULL;
+    real *dst = field == OR_F_X ? s->x : field == OR_F_V ? s->v : field == OR_F_VISC_DIFF ? s->wvd : NULL;
     if (!dst) return -1;
     for (int i = 0; i < 3 * s->n; i++) dst[i] = (real)in[i];
     return 0;

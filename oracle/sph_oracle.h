@@ -93,7 +93,8 @@ enum {
     OR_F_X = 0, OR_F_V = 1, OR_F_ACC = 2, OR_F_VOLUME = 3, OR_F_DENSITY = 4,
     OR_F_DFSPH_FACTOR = 5, OR_F_DENSITY_ADV = 6, OR_F_STRAIN_NORM = 7,
     OR_F_VISCOSITY = 8, OR_F_ROTATION = 9, OR_F_STRESS = 10, OR_F_F = 11,
-    OR_F_REST_VOLUME = 12, OR_F_X0 = 13
+    OR_F_REST_VOLUME = 12, OR_F_X0 = 13,
+    OR_F_VISC_DIFF = 16 /* x - v of the previous Weiler solve (the warm start's direction) */
 };
 
 typedef struct or_sim or_sim;

@@ -3,12 +3,15 @@
     plus the boundary, Cross model eta0 = 50, Weiler CG, dt = 5e-4, 200 steps;
   * elastic block dropped on a plate: 98,304 elastic particles (48 x 32 x 64)
     and a 200 x 200 static plate, dt = 2e-4, 500 steps.
-At full size the first steps are checked against the FP32 oracle (neighbor
+At full size the first two steps are checked against the FP32 oracle (neighbor
 lists equal, fields bit for bit, or within 1e-5 where the Cross model's powf
-enters), then the whole run through what does not depend on the size: the
-density error each step against the solver's stopping rule, no non-finite
-state, every particle inside the domain, the dropped body falling and coming to
-rest on the plate with its rest-neighbor distances kept."""
+enters), and again two steps late in the run (after step 150 of the dam break,
+400 of the elastic drop) with the oracle continuing from the GPU's state; the
+whole run through what does not depend on the size: no non-finite state, the
+realized compression of the fluid (mean of max(rho / rho0 - 1, 0) from the
+downloaded densities) within 0.1%, every particle inside the domain, the
+dropped body falling and coming to rest on the plate with its rest-neighbor
+distances kept."""
 import numpy as np
 import pytest
 
@@ -26,6 +29,31 @@ def inside(x, p):
     lo = np.asarray(p.domain_start) - 1e-6
     hi = np.asarray(p.domain_end) + 1e-6
     return bool(np.all(x >= lo) and np.all(x <= hi))
+
+
+def late_parity(ctx, sc, fields, rtol):
+    """Late in a run: the FP32 oracle, initialized on the scene's initial
+    positions (rest state, static volumes), continues from the context's x, v
+    and Weiler warm-start direction; after two more steps of both, the neighbor
+    lists are equal, the solvers ran the same iterations and the fields are
+    equal (rtol 0) or within rtol (the Cross model's powf)."""
+    orc = O.OracleSim.from_scene(sc)
+    orc.init()
+    orc.set(O.F_X, ctx.download(L.FIELD_X))
+    orc.set(O.F_V, ctx.download(L.FIELD_V))
+    orc.set(O.F_VISC_DIFF, ctx.download(L.FIELD_VISC_DIFF))
+    for _ in range(2):
+        sg, so = ctx.step(), orc.step()
+        assert (sg["iterations_v"], sg["iterations_p"], sg["iterations_cg"]) == (
+            so["iterations_v"], so["iterations_p"], so["iterations_cg"])
+    assert_lists_equal(ctx, orc)
+    for name, fg, fo in fields:
+        b = orc.get(fo)
+        a = gpu_field(ctx, fg, b.shape)
+        if rtol == 0.0:
+            assert np.array_equal(a, b), name
+        else:
+            assert np.allclose(a, b, rtol=rtol, atol=rtol * (np.abs(b).max() + 1e-30)), name
 
 
 @pytest.mark.timeout(400)
@@ -48,10 +76,18 @@ def test_nonnewton_dam_full_size():
     fl = sc.kind == KIND_FLUID
     y0 = sc.x[fl, 1].mean()
     for step in range(2, 200):
+        if step == 150:
+            late_parity(ctx, sc, fields=FIELDS, rtol=1e-5)
+            continue  # late_parity stepped the context twice
+        if step == 151:
+            continue
         st = ctx.step()
         assert not (st["status"] & 8), step  # non-finite state
-        assert st["avg_density_err_p"] <= p.max_error * 0.01 * rho0 or st["iterations_p"] == p.max_iterations, step
         assert st["iterations_cg"] <= p.weiler_max_iter
+        if step % 50 == 49:
+            # realized compression, from the densities the step computed
+            rho = ctx.download(L.FIELD_DENSITY)[fl].astype(np.float64)
+            assert np.mean(np.maximum(rho / rho0 - 1.0, 0.0)) <= 1e-3, step
     x = ctx.download(L.FIELD_X).reshape(-1, 3)
     assert np.all(np.isfinite(x)) and inside(x, p)
     assert x[fl, 1].mean() < y0 - 1e-3  # the column has started to collapse
@@ -77,6 +113,10 @@ def test_elastic_drop_full_size():
     x0 = sc.x[el].astype(np.float64)
     ys = []
     for step in range(2, 500):
+        if step == 400:
+            late_parity(ctx, sc, fields=FIELDS + ELASTIC_FIELDS, rtol=0.0)
+        if step in (400, 401):
+            continue
         st = ctx.step()
         assert not (st["status"] & 8), step
         if step % 50 == 49:

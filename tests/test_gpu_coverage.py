@@ -1,0 +1,264 @@
+"""GPU parity for the paths the other GPU tests leave out:
+
+- the four non-Newtonian models Casson, Carreau, Bingham and Herschel-Bulkley
+  (NonNewton, SPH/main.py:422-459) through the Weiler CG;
+- the boundary viscosity term and its reaction on a dynamic solid
+  (ViscosityModel, :1446-1460), explicit and beside the Weiler solve;
+- the DFSPH pressure and divergence solves run for several iterations every
+  step and stopped at their limit (:1683-1760, :1762-1841), on a fluid block
+  thrown at the floor;
+- a run continued from another's state (x, v and the Weiler warm start's
+  direction) equal bit for bit to the uninterrupted run;
+- the 64-scene mixed batch at full size in one pool;
+- the serving loop on request sizes of the serving stream, the largest above
+  200k particles.
+"""
+import numpy as np
+import pytest
+
+from nonnewton_amd import _lib as L
+from nonnewton_amd import scenes
+from nonnewton_amd.params import KIND_ELASTIC, KIND_FLUID, VISCOSITY_EXPLICIT, VISCOSITY_WEILER, SceneParams
+from nonnewton_amd.scenes import SPACING, Request, assemble, jitter, lattice, shell
+from oracle import oracle as O
+
+from test_gpu_parity import ELASTIC_FIELDS, FIELDS, assert_fields_equal, assert_lists_equal, gpu_field, pair
+
+pytestmark = pytest.mark.gpu
+
+
+def nonnewton_scene(method, **kw):
+    base = dict(use_non_newtonian=True, non_newton_method=method, viscosity0=50.0, viscosity_inf=0.1,
+                consistency_index=1.0, power_index=0.7, viscosity_method=VISCOSITY_WEILER, weiler_tol=1e-4, dt=5e-4)
+    base.update(kw)
+    return scenes.tiny_box(fluid_counts=(8, 8, 8), cells=(16, 16, 16), **base)
+
+
+# parameters that put the box's strain rates (0.016-0.115 1/s over the first
+# steps) on both sides of the Bingham / Herschel-Bulkley switch
+MODEL_PARAMS = {
+    3: dict(mu_c=0.5, yield_stress=2.0, max_viscosity=1e3),   # Casson
+    4: dict(),                                                # Carreau
+    5: dict(critical_strain_rate=0.05),                       # Bingham
+    6: dict(critical_strain_rate=0.05, consistency_index=5.0, power_index=0.6),  # Herschel-Bulkley
+}
+
+
+@pytest.mark.parametrize("method", sorted(MODEL_PARAMS))
+def test_models_3_to_6_match_oracle(method):
+    """The same CG iteration counts as the FP32 oracle at each of 4 steps and
+    the fields within 1e-5 (the models' powf / sqrtf chains may round
+    differently in the device and host libraries)."""
+    sc = nonnewton_scene(method, **MODEL_PARAMS[method])
+    ctx, orc = pair(sc)
+    for step in range(4):
+        sg, so = ctx.step(), orc.step()
+        assert (sg["iterations_v"], sg["iterations_p"], sg["iterations_cg"]) == \
+            (so["iterations_v"], so["iterations_p"], so["iterations_cg"]), step
+    assert so["iterations_cg"] > 0
+    eta = orc.get(O.F_VISCOSITY)[sc.kind == KIND_FLUID]
+    if method in (5, 6):  # both branches of the switch were taken
+        assert eta.max() == pytest.approx(50.0) and eta.min() < 49.0
+    for name, fg, fo in FIELDS[:3] + [("viscosity", L.FIELD_VISCOSITY, O.F_VISCOSITY),
+                                      ("strain", L.FIELD_STRAIN_NORM, O.F_STRAIN_NORM)]:
+        b = orc.get(fo)
+        a = gpu_field(ctx, fg, b.shape)
+        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-12), name
+
+
+def fluid_against_elastic(seed=21, **kw):
+    """A jittered 8^3 fluid block against a 4 x 8 x 8 elastic block, one
+    spacing apart, in a walled box: fluid-elastic pairs from the first step."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cl = (20, 16, 14)
+    p = SceneParams(domain_end=tuple(c * SPACING for c in cl), gravity=(0.0, -9.81, 0.0), dt=5e-4,
+                    youngs_modulus=1e5, poisson_ratio=0.33, **kw)
+    fl = jitter(rng, lattice((3 * SPACING, 3 * SPACING, 3 * SPACING), (8, 8, 8)))
+    el = jitter(rng, lattice((11 * SPACING, 3 * SPACING, 3 * SPACING), (4, 8, 8)))
+    return assemble("fluid_against_elastic", p, fl, shell(cl), el, 1)
+
+
+@pytest.mark.parametrize("method", [VISCOSITY_EXPLICIT, VISCOSITY_WEILER])
+def test_boundary_viscosity_bit_exact(method):
+    """boundaryViscosity != 0 in a fluid scene with walls and an elastic block:
+    the boundary term on the fluid and its reaction on the elastic particles,
+    with the explicit fluid term or beside the (Newtonian) Weiler solve."""
+    sc = fluid_against_elastic(boundary_viscosity=0.05, viscosity_method=method,
+                               use_non_newtonian=method == VISCOSITY_WEILER, viscosity0=2.0, viscosity=0.02)
+    ctx, orc = pair(sc)
+    for step in range(6):
+        sg, so = ctx.step(), orc.step()
+        assert (sg["iterations_v"], sg["iterations_p"], sg["iterations_cg"]) == \
+            (so["iterations_v"], so["iterations_p"], so["iterations_cg"]), step
+    assert_fields_equal(ctx, orc, FIELDS + ELASTIC_FIELDS[:2])
+    # the reaction reached the elastic block
+    el = sc.kind == KIND_ELASTIC
+    sc0 = sc.params.replace(boundary_viscosity=0.0)
+    ref = O.OracleSim(sc0, sc.x, sc.kind, sc.v)
+    ref.init()
+    for _ in range(6):
+        ref.step()
+    assert not np.array_equal(ref.get(O.F_V)[el], orc.get(O.F_V)[el])
+
+
+def impact_block(r, vy, divergence, max_error, max_iterations=100, seed=0):
+    """An 8^3 fluid block thrown at the floor at vy m/s, its particle radius a
+    little above half the lattice spacing (the rest state is over-dense):
+    the pressure solve iterates every step."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cl = (16, 20, 16)
+    p = SceneParams(domain_end=tuple(c * SPACING for c in cl), gravity=(0.0, -9.81, 0.0), dt=1e-3,
+                    particle_radius=r, enable_divergence_solver=divergence, max_error=max_error,
+                    max_iterations=max_iterations)
+    fl = jitter(rng, lattice((3 * SPACING, 2.6 * SPACING, 3 * SPACING), (8, 8, 8)))
+    sc = assemble("impact_block", p, fl, shell(cl), None, 1)
+    sc.v[: len(fl), 1] = vy
+    return sc
+
+
+IMPACTS = {
+    # pressure solve 3-6 iterations every step, divergence solve off
+    "pressure": dict(r=0.011, vy=-6.0, divergence=False, max_error=0.02),
+    # divergence solve 3-4 iterations every step
+    "divergence": dict(r=0.0108, vy=-4.0, divergence=True, max_error=0.01),
+    # the pressure solve stopped at a limit of 4 iterations (NN_W_NOCONV)
+    "limit": dict(r=0.011, vy=-6.0, divergence=False, max_error=0.02, max_iterations=4),
+}
+
+
+@pytest.mark.parametrize("case", list(IMPACTS))
+def test_solvers_iterate_bit_exact(case):
+    """Every field bit for bit against the FP32 oracle at each of 12 steps in
+    which a DFSPH solver runs several iterations: the solver loop's chunks,
+    its iteration limit and the iterated updates (:1683-1760, :1762-1841)."""
+    sc = impact_block(**IMPACTS[case])
+    ctx, orc = pair(sc)
+    its = []
+    for step in range(12):
+        sg, so = ctx.step(), orc.step()
+        assert (sg["iterations_v"], sg["iterations_p"]) == (so["iterations_v"], so["iterations_p"]), step
+        # no error bits (32 is the step's "a solver stopped at its limit" bit)
+        assert (sg["status"] & ~32) == 0 and so["status"] == 0, step
+        # FP64 sums in another order: equal to rounding
+        assert sg["avg_density_err_p"] == pytest.approx(so["avg_density_err_p"], rel=1e-9, abs=1e-9), step
+        assert_fields_equal(ctx, orc, FIELDS)
+        its.append((sg["iterations_v"], sg["iterations_p"], sg["rc"]))
+    assert_lists_equal(ctx, orc)
+    iv = [a for a, _, _ in its]
+    ip = [b for _, b, _ in its]
+    if case == "pressure":
+        assert min(ip) >= 3 and max(ip) >= 5
+    elif case == "divergence":
+        assert min(iv) >= 2 and max(iv) >= 4
+    else:
+        assert max(ip) == 4 and any(rc == L.NN_W_NOCONV for _, _, rc in its)
+
+
+def test_continued_run_equals_uninterrupted():
+    """A run resumed in a new context from another's x, v and Weiler warm-start
+    direction after 3 steps equals the uninterrupted run after 6 steps, bit for
+    bit (the rest of the state -- static volumes, rest lists -- comes from init
+    on the original positions)."""
+    sc = nonnewton_scene(0, viscosity0=20.0)
+    a = L.Context(sc.params, sc.n)
+    a.upload(sc.x, sc.kind, sc.v)
+    a.init()
+    a.step(3)
+    state = {f: a.download(f) for f in (L.FIELD_X, L.FIELD_V, L.FIELD_VISC_DIFF)}
+    assert np.abs(state[L.FIELD_VISC_DIFF]).max() > 0
+    a.step(3)
+    b = L.Context(sc.params, sc.n)
+    b.upload(sc.x, sc.kind, sc.v)
+    b.init()
+    for f, arr in state.items():
+        b.upload_field(f, arr)
+    b.step(3)
+    for f in (L.FIELD_X, L.FIELD_V, L.FIELD_DENSITY, L.FIELD_VISC_DIFF):
+        assert np.array_equal(a.download(f), b.download(f)), f
+    a.close()
+    b.close()
+
+
+def alone(sc, steps, fields=(L.FIELD_X, L.FIELD_V, L.FIELD_DENSITY)):
+    c = L.Context(sc.params, sc.n)
+    c.upload(sc.x, sc.kind, sc.v)
+    c.init()
+    st = None
+    for _ in range(steps):
+        st = c.step()
+    out = {f: c.download(f) for f in fields}
+    c.close()
+    return out, st
+
+
+@pytest.mark.timeout(600)
+def test_mixed_batch_64_scenes():
+    """All 64 scenes of the mixed batch (seed 3, 1.6M particles) in one pool
+    for 6 ticks: no error status, finite state, each scene's average density
+    error of the pressure solve within 0.1% of rho0; after 2 ticks two
+    Newtonian scenes with elastic blocks equal the FP32 oracle bit for bit and
+    a power-law scene within 1e-5; after 6 ticks the scenes that set the CG tail
+    (13, 33, 34: power law with n < 0.7) equal themselves stepped alone."""
+    batch = scenes.mixed_batch()
+    assert len(batch) == 64
+    pool = L.Pool(200 << 30)
+    slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in batch]
+    assert None not in slots
+    exact, close = (0, 50), (13,)
+    for tick in range(6):
+        rc, stepped = pool.tick()
+        assert rc in (L.NN_OK, L.NN_W_NOCONV) and stepped == 64, tick
+        st = pool.stats_array(slots)
+        assert np.all((st["status"] & 15) == 0), tick
+        rho0 = np.array([s.params.density0 for s in batch])
+        assert np.all(st["avg_density_err_p"] <= 1e-3 * rho0), tick
+        if tick == 1:
+            for k in exact + close:
+                orc = O.OracleSim.from_scene(batch[k])
+                orc.init()
+                for _ in range(2):
+                    so = orc.step()
+                assert (st["iterations_cg"][k], st["iterations_p"][k]) == (so["iterations_cg"], so["iterations_p"]), k
+                for name, fg, fo in FIELDS[:5]:
+                    b = orc.get(fo)
+                    a = pool.read(slots[k], fg).astype(np.float64).reshape(b.shape)
+                    if k in exact:
+                        assert np.array_equal(a, b), (k, name)
+                    else:
+                        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-12), (k, name)
+    for k in range(64):
+        x = pool.read(slots[k], L.FIELD_X)
+        assert np.all(np.isfinite(x)), k
+    for k in (13, 33, 34):
+        ref, st = alone(batch[k], 6)
+        for f, arr in ref.items():
+            assert np.array_equal(pool.read(slots[k], f), arr), (k, f)
+        assert pool.stats(slots[k])["iterations_cg"] == st["iterations_cg"]
+    pool.close()
+
+
+@pytest.mark.timeout(600)
+def test_serving_stream_real_sizes():
+    """The serving stream's first 24 requests (seed 4: 1k-500k particles, the
+    largest above 200k) at 20 requests/s, each cut to 3 steps, through the
+    serving loop under the 200 GB budget; the largest, the smallest and one
+    elastic request give the same bits as when stepped alone."""
+    reqs = scenes.serving_requests(count=24)
+    reqs = [Request(q.rid, q.arrival, q.scene, 3) for q in reqs]
+    sizes = [q.scene.n for q in reqs]
+    assert max(sizes) >= 200_000
+    from nonnewton_amd.serving import ServingLoop
+
+    loop = ServingLoop(200 << 30, keep_results=True)
+    rep = loop.run(reqs)
+    loop.close()
+    assert all(r.finished is not None and not r.failed for r in rep.records)
+    assert all(r.done_steps == 3 for r in rep.records)
+    assert rep.peak_bytes <= 200 << 30 and rep.peak_resident >= 2
+    big = int(np.argmax(sizes))
+    small = int(np.argmin(sizes))
+    el = next(k for k, q in enumerate(reqs) if q.scene.n_elastic > 0 and k not in (big, small))
+    for k in (big, small, el):
+        ref, _ = alone(reqs[k].scene, 3, fields=(L.FIELD_X, L.FIELD_V))
+        assert np.array_equal(rep.records[k].result["x"], ref[L.FIELD_X]), k
+        assert np.array_equal(rep.records[k].result["v"], ref[L.FIELD_V]), k

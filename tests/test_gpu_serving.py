@@ -70,3 +70,28 @@ def test_serving_budget_queues_requests():
     assert all(r.finished is not None and not r.failed for r in rep.records)
     for rec, q in zip(rep.records, reqs):
         assert np.array_equal(rec.result["x"], alone(q.scene, q.steps)["x"])
+
+
+def test_serving_rejects_what_cannot_be_admitted():
+    """A request larger than the budget allows and one with a particle outside
+    its grid are marked failed with a reason; the loop returns and the other
+    requests are served as when alone."""
+    reqs = requests(steps=(2, 2, 2, 2), seed=6)
+    pool = L.Pool(1 << 34)
+    fp = max(pool.footprint(q.scene.params, q.scene.n, q.scene.n_elastic) for q in reqs)
+    pool.close()
+    big = scenes.mixed_scene(np.random.Generator(np.random.PCG64(9)), 60000, 0, 1.0, 1.0, False, "big")
+    reqs[1] = Request(1, reqs[1].arrival, big, 2)
+    bad = reqs[2].scene
+    x = bad.x.copy()
+    x[0] = (-1.0, 0.1, 0.1)  # below the domain: out of grid at admission
+    bad.x = x
+    loop = ServingLoop(int(2.5 * fp), keep_results=True)
+    rep = loop.run(reqs)
+    loop.close()
+    r = rep.records
+    assert r[1].failed and "budget" in r[1].error and r[1].slot is None
+    assert r[2].failed and r[2].error and r[2].slot is None
+    for k in (0, 3):
+        assert not r[k].failed and r[k].done_steps == 2
+        assert np.array_equal(r[k].result["x"], alone(reqs[k].scene, 2)["x"])

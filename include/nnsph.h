@@ -170,6 +170,25 @@ int32_t nn_upload_field(nn_ctx *ctx, int32_t field, const void *host, size_t byt
  * -1 padded, original indices, in the reference's serial list order. */
 int32_t nn_neighbors(nn_ctx *ctx, int32_t *nbr, int32_t *count, int32_t cap);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The lists of nn_neighbors -- meta.ns.run_search() +
+ * meta.ns.neighbors (:815-847), same sets and serial order -- for n positions
+ * already in device memory (x: n x 3 float, caller order), written to device
+ * memory: nbr int32[n][cap] (-1 padded, original indices), count int32[n],
+ * status int32[1] (bits of nn_step_stats.status: 1 neighbor overflow, 2 cell
+ * overflow, 4 out of grid; an out-of-grid particle gets no list and is nobody's
+ * neighbor).  Only params' domain and particle_radius are read.  Enqueued on
+ * `stream` (a hipStream_t; NULL = the default stream) and asynchronous: the
+ * outputs, status included, are valid once the stream reaches that point.
+ * Allocates nothing: the search's scratch is one fixed __device__ array of
+ * 42 MB per GPU, held from the first call on that GPU until the process ends;
+ * n above 1048576 or a grid above 4194304 cells is NN_E_ARG.  Every call on a
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (on different
+ * streams or threads) race. */
+int32_t nn_neighbors_noalloc(const nn_params *params, const float *x, int64_t n, int32_t *nbr, int32_t *count,
+                             int32_t cap, int32_t *status, void *stream);
+
 /* Replaces the 0-d fields meta.parm.dt[None] / viscosity[None] (:611-612,
  * :620-621) that the reference mutates at run time (:1905).  key: "dt",
  * "viscosity", "boundary_viscosity", "max_error", "max_error_v". */

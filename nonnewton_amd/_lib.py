@@ -148,6 +148,7 @@ def load():
         "nn_download": (i32, [vp, i32, vp, sz]),
         "nn_upload_field": (i32, [vp, i32, vp, sz]),
         "nn_neighbors": (i32, [vp, P(i32), P(i32), i32]),
+        "nn_neighbors_noalloc": (i32, [P(NnParams), vp, i64, vp, vp, i32, vp, vp]),
         "nn_set_param": (i32, [vp, ctypes.c_char_p, ctypes.c_double]),
         "nn_counts": (i32, [vp, P(i64), P(i64)]),
         "nn_set_block": (i32, [vp, i32]),

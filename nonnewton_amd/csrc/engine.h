@@ -201,6 +201,9 @@ struct DevRef {
 void launch_fill_i32(int* p, long n, int v, hipStream_t st);
 void launch_fill_f4(float4* p, long n, float4 v, hipStream_t st);
 void launch_scan(const DevRef& d, int n, hipStream_t st);  // cell_start = exclusive scan(cell_count)
+// exclusive scan of in[0..n) into out; tmp holds ceil(n / 2048) ints
+void launch_scan_i32(const int* in, int n, int* tmp, int* out, hipStream_t st);
+constexpr int SCAN_TILE_ITEMS = 2048;
 
 void launch_cell_count(const DevRef& d, int block, hipStream_t st);
 void launch_bucket(const DevRef& d, int block, hipStream_t st);

@@ -115,11 +115,16 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(const int* in, int n, cons
   }
 }
 
-void launch_scan(const DevRef& d, int n, hipStream_t st) {
+// exclusive scan of in[0..n) into out, with ceil(n / SCAN_TILE) ints of tmp
+void launch_scan_i32(const int* in, int n, int* tmp, int* out, hipStream_t st) {
   int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, d.h.cell_count, n, d.h.scan_tmp);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, d.h.scan_tmp, nb);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, st, d.h.cell_count, n, d.h.scan_tmp, d.h.cell_start);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, tmp, nb);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp, out);
+}
+
+void launch_scan(const DevRef& d, int n, hipStream_t st) {
+  launch_scan_i32(d.h.cell_count, n, d.h.scan_tmp, d.h.cell_start, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -240,9 +240,10 @@ def test_mixed_batch_64_scenes():
 @pytest.mark.timeout(600)
 def test_serving_stream_real_sizes():
     """The serving stream's first 24 requests (seed 4: 1k-500k particles, the
-    largest above 200k) at 20 requests/s, each cut to 3 steps, through the
-    serving loop under the 200 GB budget; the largest, the smallest and one
-    elastic request give the same bits as when stepped alone."""
+    largest above 200k), each cut to 3 steps and arriving 100 times faster than
+    at 20 requests/s (so that requests overlap), through the serving loop under
+    the 200 GB budget; the largest, the smallest and one elastic request give
+    the same bits as when stepped alone."""
     reqs = scenes.serving_requests(count=24)
     reqs = [Request(q.rid, q.arrival, q.scene, 3) for q in reqs]
     sizes = [q.scene.n for q in reqs]
@@ -250,7 +251,7 @@ def test_serving_stream_real_sizes():
     from nonnewton_amd.serving import ServingLoop
 
     loop = ServingLoop(200 << 30, keep_results=True)
-    rep = loop.run(reqs)
+    rep = loop.run(reqs, time_scale=0.01)
     loop.close()
     assert all(r.finished is not None and not r.failed for r in rep.records)
     assert all(r.done_steps == 3 for r in rep.records)

@@ -16,8 +16,10 @@ Weiler implicit solver, 25% of scenes with an elastic block, dt 5e-4.
   roofline  the kernel with the largest share of a probe tick (on a separate
             pool): algorithmic bytes per launch (sweep table below, from the
             logged per-scene counts and iteration numbers) over its mean
-            launch duration, from HIP events around every 8th launch of it in
-            the timed region; also a whole CG iteration and the whole tick
+            launch duration, from HIP events carried by every 8th launch of it
+            in the timed region, less the launch and event latency an empty
+            kernel shows when bracketed the same way; also a whole CG
+            iteration and the whole tick
   serving   continuous batching of the Poisson request stream (1k-500k
             particles, 50-500 steps) at 5, 20 and 80 requests/s
   cpu       the oracle (FP32, the reference's arithmetic, OpenMP over particles)
@@ -633,11 +635,17 @@ def main():
     if times:
         kb = {k: sum(kernel_algorithmic_bytes(k, sc, st) or 0 for ss in stats for sc, st in zip(scs, ss))
               for k in times}
-        avg = {k: (ms * 1e-3 / timed) for k, (ms, timed, _) in times.items() if timed}
+        # a timed launch's interval holds the launch and event latency as well
+        # as the kernel: measured on an empty kernel bracketed the same way
+        ovh = L.event_overhead_ms(200) * 1e-3
+        raw = {k: (ms * 1e-3 / timed) for k, (ms, timed, _) in times.items() if timed}
+        avg = {k: max(v - ovh, 1e-9) for k, v in raw.items()}
         name = b["dominant"]
         if name in avg and kernel_algorithmic_bytes(name, scs[0], stats[0][0]) is not None:
             roof = roofline_entry(name, kb[name], avg[name], times[name][2], elapsed, args.workload)
             roof["timed_launches"] = times[name][1]
+            roof["event_interval_us"] = round(1e6 * raw[name], 2)
+            roof["event_overhead_us"] = round(1e6 * ovh, 2)
         if "cg_matvec" in avg and "cg_update" in avg:
             t_it = avg["cg_matvec"] * times["cg_matvec"][2] + avg["cg_update"] * times["cg_update"][2]
             by = kb["cg_matvec"] + kb["cg_update"]

@@ -267,6 +267,12 @@ int32_t nn_pool_profile(nn_pool *pool, int32_t enable);
 int32_t nn_pool_profile_mask(nn_pool *pool, uint32_t mask, int32_t stride);
 int32_t nn_pool_kernel_times(nn_pool *pool, double *ms, int64_t *launches, int32_t n);
 int32_t nn_pool_kernel_launches(nn_pool *pool, int64_t *seen, int32_t n);
+/* The launch and event latency contained in a timed launch's interval (no
+ * counterpart in the reference): the mean interval, in ms, between the start
+ * and stop events of an empty kernel bracketed as the timer brackets a timed
+ * launch, over `reps` launches on a stream of its own.  A kernel's duration is
+ * its timed interval minus this. */
+int32_t nn_event_overhead(int32_t reps, double *ms);
 
 /* ------------------------------------------------------------------------
  * Surface reconstruction: the particle-to-mesh path of SRTool

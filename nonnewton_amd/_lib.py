@@ -178,6 +178,7 @@ def load():
         "nn_pool_profile_mask": (i32, [vp, ctypes.c_uint32, i32]),
         "nn_pool_kernel_launches": (i32, [vp, P(i64), i32]),
         "nn_pool_kernel_times": (i32, [vp, P(ctypes.c_double), P(i64), i32]),
+        "nn_event_overhead": (i32, [i32, P(ctypes.c_double)]),
         "nn_surface_create": (i32, [P(NnSurfaceParams), P(vp)]),
         "nn_surface_destroy": (i32, [vp]),
         "nn_surface_set_budget": (i32, [vp, i64]),
@@ -200,6 +201,13 @@ def load():
 
 
 KERNEL_COUNT = 21
+
+
+def event_overhead_ms(reps: int = 200) -> float:
+    """Launch and event latency inside a timed launch's interval (nn_event_overhead)."""
+    ms = ctypes.c_double()
+    check(load().nn_event_overhead(int(reps), ctypes.byref(ms)), "nn_event_overhead")
+    return float(ms.value)
 
 
 def device_name() -> str:

@@ -5,6 +5,8 @@
 #include <mutex>
 #include <thread>
 
+#include <hip/hip_ext.h>
+
 #include "batch.h"
 
 namespace nn {
@@ -259,6 +261,37 @@ static int copy_seen(const nn::KernelTimer& t, int64_t* seen, int32_t n) {
 }
 
 const char* nn_kernel_name(int32_t id) { return nn::kernel_name(id); }
+
+__global__ void k_empty(int) {}
+
+// The launch and event latency inside a timed launch's interval: an empty
+// kernel launched behind another one and bracketed by a start and a stop
+// event exactly as the per-kernel timer brackets a timed launch, `reps`
+// times on a stream of its own; the mean interval in ms.
+int32_t nn_event_overhead(int32_t reps, double* ms) {
+  if (reps <= 0 || reps > 100000 || !ms) return NN_E_ARG;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double total = 0.0;
+  for (int r = -8; r < reps && e == hipSuccess; r++) {  // 8 untimed rounds first
+    hipLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, st, 0);
+    hipExtLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, st, e0, e1, 0u, 1);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (r >= 0) total += t;
+  }
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return nn::hip_fail(e, "nn_event_overhead");
+  *ms = total / reps;
+  return NN_OK;
+}
 
 int32_t nn_profile(nn_ctx* ctx, int32_t enable) { return nn_profile_mask(ctx, enable ? 0xFFFFFFFFu : 0u, 1); }
 

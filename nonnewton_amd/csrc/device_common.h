@@ -111,9 +111,9 @@ __device__ __forceinline__ void for_neighbors(const Dev& d, int i, int cnt, Load
 #define DEPTH_LAUNCH(kern, depth, grid, blk, st, ...)                          \
   do {                                                                        \
     switch (depth) {                                                          \
-      case 4: hipLaunchKernelGGL(kern<4>, grid, blk, 0, st, __VA_ARGS__); break; \
-      case 16: hipLaunchKernelGGL(kern<16>, grid, blk, 0, st, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL(kern<8>, grid, blk, 0, st, __VA_ARGS__); break; \
+      case 4: NN_LAUNCH(kern<4>, grid, blk, 0, st, __VA_ARGS__); break;        \
+      case 16: NN_LAUNCH(kern<16>, grid, blk, 0, st, __VA_ARGS__); break;      \
+      default: NN_LAUNCH(kern<8>, grid, blk, 0, st, __VA_ARGS__); break;       \
     }                                                                         \
   } while (0)
 

@@ -22,6 +22,7 @@
 //    key = cell_begin + (cx * gn1 + cy) * gn2 + cz; the extra last key parks
 //    particles that left the grid (reported as an error, never searched).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -197,6 +198,28 @@ struct Dev {
 struct DevRef {
   const Dev& h;
 };
+
+// Start and stop events for the next kernel launch, set by the per-kernel
+// timer around a timed launch.  The launch carries them (hipExtLaunchKernel
+// stamps them with the dispatch's own start and end times), so timing puts no
+// packets between kernels; the first launch after they are set takes them.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool launched = false;
+};
+LaunchEvents& launch_events();  // of the calling thread
+
+#define NN_LAUNCH(kern, grid, blk, shm, st, ...)                                                   \
+  do {                                                                                          \
+    LaunchEvents& _le = launch_events();                                                        \
+    if (_le.start) {                                                                            \
+      hipExtLaunchKernelGGL(kern, grid, blk, shm, st, _le.start, _le.stop, 0u, __VA_ARGS__);   \
+      _le.start = _le.stop = nullptr;                                                           \
+      _le.launched = true;                                                                      \
+    } else {                                                                                    \
+      hipLaunchKernelGGL(kern, grid, blk, shm, st, __VA_ARGS__);                                \
+    }                                                                                           \
+  } while (0)
 
 void launch_fill_i32(int* p, long n, int v, hipStream_t st);
 void launch_fill_f4(float4* p, long n, float4 v, hipStream_t st);

@@ -573,16 +573,16 @@ __global__ void k_advect(Dev d) {
 static inline dim3 grid_of(const DevRef& d, int block) { return dim3((d.h.Pn + block - 1) / block); }
 
 void launch_step_begin(const DevRef& d, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, d.h);
+  NN_LAUNCH(k_step_begin, dim3(1), dim3(256), 0, st, d.h);
 }
 void launch_clear_acc(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_clear_acc, grid_of(d, block), dim3(block), 0, st, d.h);
 }
 void launch_density_alpha(const DevRef& d, int block, int depth, hipStream_t st) {
   DEPTH_LAUNCH(k_density_alpha, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_solver_begin(const DevRef& d, int which, hipStream_t st) {
-  hipLaunchKernelGGL(k_solver_begin, dim3(1), dim3(256), 0, st, d.h, which);
+  NN_LAUNCH(k_solver_begin, dim3(1), dim3(256), 0, st, d.h, which);
 }
 void launch_div_update(const DevRef& d, int block, int depth, hipStream_t st) {
   DEPTH_LAUNCH(k_div_update, depth, grid_of(d, block), dim3(block), st, d.h);
@@ -600,10 +600,10 @@ void launch_press_update(const DevRef& d, int block, int depth, hipStream_t st) 
   DEPTH_LAUNCH(k_press_update, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_advect(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_advect, grid_of(d, block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_advect, grid_of(d, block), dim3(block), 0, st, d.h);
 }
 void launch_scene_finish(const DevRef& d, int which, hipStream_t st) {
-  if (d.h.nscenes > 0) hipLaunchKernelGGL(k_scene_finish, dim3(d.h.nscenes), dim3(FINISH_THREADS), 0, st, d.h, which);
+  if (d.h.nscenes > 0) NN_LAUNCH(k_scene_finish, dim3(d.h.nscenes), dim3(FINISH_THREADS), 0, st, d.h, which);
 }
 
 }  // namespace nn

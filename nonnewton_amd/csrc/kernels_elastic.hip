@@ -176,15 +176,15 @@ __global__ void k_elastic_force(Dev d) {
 }
 
 void launch_elastic_init(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_elastic_init, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_elastic_init, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_elastic_rot_stress(const DevRef& d, int block, hipStream_t st) {
   if (d.h.NSn <= 0) return;
-  hipLaunchKernelGGL(k_elastic_rot_stress, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_elastic_rot_stress, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_elastic_force(const DevRef& d, int block, hipStream_t st) {
   if (d.h.NSn <= 0) return;
-  hipLaunchKernelGGL(k_elastic_force, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_elastic_force, dim3((d.h.NSn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 
 }  // namespace nn

@@ -30,12 +30,12 @@ __global__ void k_fill_f4(float4* p, long n, float4 v) {
 
 void launch_fill_i32(int* p, long n, int v, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  NN_LAUNCH(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
 void launch_fill_f4(float4* p, long n, float4 v, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill_f4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  NN_LAUNCH(k_fill_f4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -118,9 +118,9 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(const int* in, int n, cons
 // exclusive scan of in[0..n) into out, with ceil(n / SCAN_TILE) ints of tmp
 void launch_scan_i32(const int* in, int n, int* tmp, int* out, hipStream_t st) {
   int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, tmp, nb);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp, out);
+  NN_LAUNCH(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp);
+  NN_LAUNCH(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, tmp, nb);
+  NN_LAUNCH(k_scan_down, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp, out);
 }
 
 void launch_scan(const DevRef& d, int n, hipStream_t st) {
@@ -317,19 +317,19 @@ __global__ void k_bvol(Dev d, int dynamic) {
 }
 
 void launch_cell_count(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_count, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_cell_count, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_bucket(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_bucket, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_place(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_place, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_place, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h);
 }
 void launch_build_lists(const DevRef& d, int block, int all, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_lists, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h, all);
+  NN_LAUNCH(k_build_lists, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h, all);
 }
 void launch_bvol(const DevRef& d, int block, int dynamic, hipStream_t st) {
-  hipLaunchKernelGGL(k_bvol, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h, dynamic);
+  NN_LAUNCH(k_bvol, dim3((d.h.Pn + block - 1) / block), dim3(block), 0, st, d.h, dynamic);
 }
 
 }  // namespace nn

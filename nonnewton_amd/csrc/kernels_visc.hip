@@ -514,31 +514,31 @@ void launch_strain(const DevRef& d, int block, int depth, hipStream_t st) {
   DEPTH_LAUNCH(k_strain, depth, grid_of(d, block), dim3(block), st, d.h);
 }
 void launch_cg_init(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d.h);
+  NN_LAUNCH(k_cg_init, grid4_of(d, block), dim3(quad_block(block)), 0, st, d.h);
 }
 void launch_cg_guess(const DevRef& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d.h);
+  NN_LAUNCH(k_cg_guess, grid_of(d, block), dim3(block), 0, st, d.h);
 }
 void launch_cg_matvec(const DevRef& d, int block, int depth, int start, hipStream_t st) {
   const dim3 g = grid4_of(d, block), b = dim3(quad_block(block));
   switch (depth) {
-    case 1: hipLaunchKernelGGL(k_cg_matvec<1>, g, b, 0, st, d.h, start); break;
-    case 4: hipLaunchKernelGGL(k_cg_matvec<4>, g, b, 0, st, d.h, start); break;
-    case 8: hipLaunchKernelGGL(k_cg_matvec<8>, g, b, 0, st, d.h, start); break;
-    default: hipLaunchKernelGGL(k_cg_matvec<2>, g, b, 0, st, d.h, start); break;
+    case 1: NN_LAUNCH(k_cg_matvec<1>, g, b, 0, st, d.h, start); break;
+    case 4: NN_LAUNCH(k_cg_matvec<4>, g, b, 0, st, d.h, start); break;
+    case 8: NN_LAUNCH(k_cg_matvec<8>, g, b, 0, st, d.h, start); break;
+    default: NN_LAUNCH(k_cg_matvec<2>, g, b, 0, st, d.h, start); break;
   }
 }
 void launch_cg_matvec_list(const DevRef& d, int depth, int k, int grid, hipStream_t st) {
   const dim3 g = dim3(grid > 0 ? grid : 1), b = dim3(MVL_THREADS);
   switch (depth) {
-    case 1: hipLaunchKernelGGL(k_cg_matvec_list<1>, g, b, 0, st, d.h, k); break;
-    case 4: hipLaunchKernelGGL(k_cg_matvec_list<4>, g, b, 0, st, d.h, k); break;
-    case 8: hipLaunchKernelGGL(k_cg_matvec_list<8>, g, b, 0, st, d.h, k); break;
-    default: hipLaunchKernelGGL(k_cg_matvec_list<2>, g, b, 0, st, d.h, k); break;
+    case 1: NN_LAUNCH(k_cg_matvec_list<1>, g, b, 0, st, d.h, k); break;
+    case 4: NN_LAUNCH(k_cg_matvec_list<4>, g, b, 0, st, d.h, k); break;
+    case 8: NN_LAUNCH(k_cg_matvec_list<8>, g, b, 0, st, d.h, k); break;
+    default: NN_LAUNCH(k_cg_matvec_list<2>, g, b, 0, st, d.h, k); break;
   }
 }
 void launch_cg_update(const DevRef& d, int nblocks, int k, hipStream_t st) {
-  if (nblocks > 0) hipLaunchKernelGGL(k_cg_update, dim3(nblocks), dim3(UPD_THREADS), 0, st, d.h, k);
+  if (nblocks > 0) NN_LAUNCH(k_cg_update, dim3(nblocks), dim3(UPD_THREADS), 0, st, d.h, k);
 }
 
 }  // namespace nn

@@ -217,12 +217,15 @@ int32_t nn_pool_admit(nn_pool *pool, const nn_params *params, const float *x, co
  * through *stepped. */
 int32_t nn_pool_tick(nn_pool *pool, nn_step_stats *stats, int32_t stats_len, int32_t *stepped);
 /* The same tick started on a library thread: returns at once, so the caller
- * can admit, retire or prepare the next requests while the GPU steps the pool
- * (calls that touch the pool wait for the tick; nn_pool_tick returns
- * NN_E_BUSY).  nn_pool_sync waits for it and returns what nn_pool_tick would
- * have: its status, the stats of the slots it stepped, the count stepped.
- * NN_E_BUSY if a tick is already in flight; nn_pool_sync without one is
- * NN_E_STATE.  (The reference steps one scene synchronously; no counterpart.) */
+ * can admit, retire or prepare the next requests while the GPU steps the pool.
+ * Admission, retirement, nn_pool_steps and nn_pool_info run during a tick (on
+ * any thread, synchronous ticks included); nn_pool_read and the launch-shape
+ * and profiling calls, which touch the batch, wait for it; nn_pool_tick
+ * returns NN_E_BUSY.  nn_pool_sync waits for it and returns what nn_pool_tick
+ * would have: its status, the stats of the slots it stepped, the count
+ * stepped.  NN_E_BUSY if a tick is already in flight; nn_pool_sync without
+ * one is NN_E_STATE.  (The reference steps one scene synchronously; no
+ * counterpart.) */
 int32_t nn_pool_tick_async(nn_pool *pool);
 int32_t nn_pool_sync(nn_pool *pool, nn_step_stats *stats, int32_t stats_len, int32_t *stepped);
 int32_t nn_pool_retire(nn_pool *pool, int32_t slot);

@@ -359,6 +359,11 @@ struct nn_pool {
   long long in_use = 0;
   std::unique_ptr<Batch> main;
   std::vector<PoolSlot> slots;
+  // tick_mu is held for a whole tick (membership change and step): taken by
+  // what touches the batch (downloads, launch shapes, profiling).  mu guards
+  // the slots, the budget and membership: admission and retirement take only
+  // mu, so they run while a tick steps the batch.  Order: tick_mu, then mu.
+  std::mutex tick_mu;
   std::mutex mu;
   int block = 256;
   int kblock[nn::K_COUNT] = {};
@@ -598,15 +603,23 @@ static int pool_apply_membership(nn_pool* p) {
 }
 
 static int pool_tick_impl(nn_pool* p, nn_step_stats* stats, int32_t stats_len, int32_t* stepped) {
-  std::lock_guard<std::mutex> lk(p->mu);
-  int rc = pool_apply_membership(p);
-  if (rc < 0) return rc;
-  if (stepped) *stepped = (int32_t)p->main->scenes.size();
-  if (p->main->scenes.empty()) return NN_OK;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
+  std::vector<int> members;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    int rc = pool_apply_membership(p);
+    if (rc < 0) return rc;
+    members = p->member_slot;
+  }
+  if (stepped) *stepped = (int32_t)members.size();
+  if (members.empty()) return NN_OK;
+  // the step runs without mu: admissions and retirements meanwhile touch only
+  // slots outside the batch or flags read at the next membership change
   std::vector<nn_step_stats> st;
-  rc = p->main->step(&st);
+  const int rc = p->main->step(&st);
+  std::lock_guard<std::mutex> lk(p->mu);
   for (size_t k = 0; k < st.size(); k++) {
-    const int s = p->member_slot[k];
+    const int s = members[k];
     p->slots[s].steps++;
     if (stats && s < stats_len) stats[s] = st[k];
   }
@@ -684,6 +697,7 @@ static Batch* pool_locate(nn_pool* p, int slot, int& index) {
 
 int32_t nn_pool_read(nn_pool* p, int32_t slot, int32_t field, void* host, size_t bytes) {
   if (!p || slot < 0 || slot >= POOL_CAPACITY || !host) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   int idx = 0;
   Batch* b = pool_locate(p, slot, idx);
@@ -713,6 +727,7 @@ int32_t nn_pool_profile(nn_pool* p, int32_t enable) { return nn_pool_profile_mas
 
 int32_t nn_pool_profile_mask(nn_pool* p, uint32_t mask, int32_t stride) {
   if (!p || stride < 1) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   p->main->timer.on = mask != 0;
   p->main->timer.mask = mask;
@@ -723,12 +738,14 @@ int32_t nn_pool_profile_mask(nn_pool* p, uint32_t mask, int32_t stride) {
 
 int32_t nn_pool_kernel_times(nn_pool* p, double* ms, int64_t* launches, int32_t n) {
   if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   return copy_times(p->main->timer, ms, launches, n);
 }
 
 int32_t nn_pool_kernel_launches(nn_pool* p, int64_t* seen, int32_t n) {
   if (!p) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   return copy_seen(p->main->timer, seen, n);
 }
@@ -737,6 +754,7 @@ int32_t nn_pool_set_block(nn_pool* p, int32_t block) {
   if (!p) return NN_E_ARG;
   if (block == 0) block = 256;
   if (block < 64 || block > 1024 || (block & (block - 1))) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   p->block = block;
   p->main->block = block;
@@ -745,6 +763,7 @@ int32_t nn_pool_set_block(nn_pool* p, int32_t block) {
 
 int32_t nn_pool_set_kernel_depth(nn_pool* p, int32_t kernel, int32_t depth) {
   if (!p || kernel < 0 || kernel >= nn::K_COUNT || !depth_ok(depth)) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   p->kdepth[kernel] = depth;
   p->main->kdepth[kernel] = depth;
@@ -754,6 +773,7 @@ int32_t nn_pool_set_kernel_depth(nn_pool* p, int32_t kernel, int32_t depth) {
 int32_t nn_pool_set_kernel_block(nn_pool* p, int32_t kernel, int32_t block) {
   if (!p || kernel < 0 || kernel >= nn::K_COUNT) return NN_E_ARG;
   if (block != 0 && (block < 64 || block > 1024 || (block & (block - 1)))) return NN_E_ARG;
+  std::lock_guard<std::mutex> tk(p->tick_mu);
   std::lock_guard<std::mutex> lk(p->mu);
   p->kblock[kernel] = block;
   p->main->kblock[kernel] = block;

@@ -185,7 +185,8 @@ def test_async_tick_equals_sync_tick():
     """nn_pool_tick_async + nn_pool_sync step the pool exactly as nn_pool_tick
     does (same fields, same stats), a second start while one is in flight is
     NN_E_BUSY, as is a synchronous tick, and a sync without a tick is
-    NN_E_STATE.  Admission while a tick is in flight waits for it."""
+    NN_E_STATE.  A scene admitted while a tick is in flight (admission runs
+    during the tick) joins at the next tick, as one admitted after it."""
     batch = scenes.mixed_batch(count=3, lo=2000, hi=5000)
     pools = [L.Pool(8 << 30), L.Pool(8 << 30)]
     slots = [[p.admit(s.params, s.x, s.kind, s.v) for s in batch[:2]] for p in pools]
@@ -198,7 +199,7 @@ def test_async_tick_equals_sync_tick():
             assert e.value.code == L.NN_E_BUSY
             r2 = pools[1].lib.nn_pool_tick(pools[1].h, None, 0, None)
             assert r2 == L.NN_E_BUSY
-        if step == 2:  # admission overlaps the running tick
+        if step == 2:  # admitted while the tick runs
             slots[1].append(pools[1].admit(batch[2].params, batch[2].x, batch[2].kind, batch[2].v))
         rc2, n2 = pools[1].sync()
         if step == 2:

@@ -13,6 +13,7 @@ Scenes (sizes in particles; spacing d = 2r = 0.02 m, h = 4r = 0.04 m):
   rigid_drop         24x12x24 = 6,912 fluid + an 8^3 rigid cube + walls, dt 5e-4
   mixed_batch        64 scenes, dynamic count log-uniform 2k-60k, seed 3
   serving_requests   Poisson request stream, 1k-500k particles, 50-500 steps, seed 4
+  impact_block       8^3 fluid block thrown at the floor (the pressure solve iterates)
 """
 from __future__ import annotations
 
@@ -299,6 +300,23 @@ def tiny_box(seed: int = 0, fluid_counts=(6, 6, 6), cells=(14, 14, 14), **kw) ->
     return assemble("tiny_box", p, fluid, walls, None, 1)
 
 
+def impact_block(r: float = 0.011, vy: float = -6.0, divergence: bool = False, max_error: float = 0.02,
+                 max_iterations: int = 100, seed: int = 0) -> Scene:
+    """An 8^3 jittered fluid block thrown at the floor of a walled box at vy m/s,
+    its particle radius a little above half the 0.02 lattice spacing (the rest
+    state is over-dense): the DFSPH pressure solve iterates every step (3-6
+    iterations with the defaults, dt 1e-3)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cl = (16, 20, 16)
+    p = SceneParams(domain_end=tuple(c * SPACING for c in cl), gravity=(0.0, -9.81, 0.0), dt=1e-3,
+                    particle_radius=r, enable_divergence_solver=divergence, max_error=max_error,
+                    max_iterations=max_iterations)
+    fl = jitter(rng, lattice((3 * SPACING, 2.6 * SPACING, 3 * SPACING), (8, 8, 8)))
+    sc = assemble("impact_block", p, fl, shell(cl), None, 1)
+    sc.v[: len(fl), 1] = vy
+    return sc
+
+
 def by_name(name: str, **kw):
     return {
         "dam_break": dam_break,
@@ -308,4 +326,5 @@ def by_name(name: str, **kw):
         "mixed_batch": mixed_batch,
         "serving_requests": serving_requests,
         "tiny_box": tiny_box,
+        "impact_block": impact_block,
     }[name](**kw)

@@ -1,0 +1,48 @@
+"""The FP32 HIP step against the FP64 oracle (tests/fp64_error.py computes the
+table; DESIGN.md holds it): per-field single-step error for the dam break,
+the non-Newtonian dam break and the elastic drop, from the initial state and
+late in a run, and the 100-step drift of the average density error."""
+import numpy as np
+import pytest
+
+import fp64_error as F
+from nonnewton_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("name", ["dam_break", "nonnewton_dam"])
+def test_fluid_single_step_error(name):
+    """Every field within 2e-5 of the largest FP64 value, one step from the
+    initial state and one step late in the run."""
+    for state, errs in F.single_step(name).items():
+        for field, e in errs.items():
+            if e["scale"] > 0:
+                assert e["max"] <= 2e-5 * e["scale"], (state, field, e)
+
+
+@pytest.mark.timeout(900)
+def test_elastic_single_step_error():
+    """Positions within 1e-7 relative and rotations within 1e-5; velocity,
+    acceleration and stress at the FP32 noise floor of the strain (position
+    differences carry ~1e-5 relative rounding, times E = 1e5 Pa): stress
+    within 1 Pa, acceleration within 0.01 m/s^2."""
+    for state, errs in F.single_step("elastic_drop").items():
+        assert errs["x"]["max"] <= 1e-7 * errs["x"]["scale"], state
+        assert errs["rotation"]["max"] <= 1e-5, state
+        assert errs["stress"]["max"] <= 1.0, state
+        assert errs["acceleration"]["max"] <= 1e-2, state
+        assert errs["v"]["max"] <= 1e-5, state
+
+
+@pytest.mark.timeout(600)
+def test_density_error_drift_over_100_steps():
+    """The pressure solve's average density error of the GPU run stays within
+    1e-3 % of rho0 of the FP64 run's at every step, and below 0.1 % of rho0 --
+    the dam break (below rest density throughout) and a fluid block thrown at
+    the floor, whose pressure solve iterates every step."""
+    for sc in (scenes.dam_break(), scenes.impact_block()):
+        d = F.drift(sc)
+        assert d["drift_pct"] <= 1e-3, (sc.name, d)
+        assert d["max_err_pct"] <= 0.1, (sc.name, d)

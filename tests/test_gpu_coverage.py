@@ -101,21 +101,6 @@ def test_boundary_viscosity_bit_exact(method):
     assert not np.array_equal(ref.get(O.F_V)[el], orc.get(O.F_V)[el])
 
 
-def impact_block(r, vy, divergence, max_error, max_iterations=100, seed=0):
-    """An 8^3 fluid block thrown at the floor at vy m/s, its particle radius a
-    little above half the lattice spacing (the rest state is over-dense):
-    the pressure solve iterates every step."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    cl = (16, 20, 16)
-    p = SceneParams(domain_end=tuple(c * SPACING for c in cl), gravity=(0.0, -9.81, 0.0), dt=1e-3,
-                    particle_radius=r, enable_divergence_solver=divergence, max_error=max_error,
-                    max_iterations=max_iterations)
-    fl = jitter(rng, lattice((3 * SPACING, 2.6 * SPACING, 3 * SPACING), (8, 8, 8)))
-    sc = assemble("impact_block", p, fl, shell(cl), None, 1)
-    sc.v[: len(fl), 1] = vy
-    return sc
-
-
 IMPACTS = {
     # pressure solve 3-6 iterations every step, divergence solve off
     "pressure": dict(r=0.011, vy=-6.0, divergence=False, max_error=0.02),
@@ -131,7 +116,7 @@ def test_solvers_iterate_bit_exact(case):
     """Every field bit for bit against the FP32 oracle at each of 12 steps in
     which a DFSPH solver runs several iterations: the solver loop's chunks,
     its iteration limit and the iterated updates (:1683-1760, :1762-1841)."""
-    sc = impact_block(**IMPACTS[case])
+    sc = scenes.impact_block(**IMPACTS[case])
     ctx, orc = pair(sc)
     its = []
     for step in range(12):

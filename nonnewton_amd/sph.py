@@ -113,8 +113,11 @@ def _rotation_matrix(angle_rad: float, axis) -> np.ndarray:
 
 
 def transform(points: np.ndarray, cfg: dict) -> np.ndarray:
-    """Rotation about the centroid then translation (transform, SPH/main.py:133-144;
-    the reference rotates about trimesh's center_mass of the point set)."""
+    """Rotation by rotationAngle degrees about rotationAxis through the point
+    set's centroid, then translation (transform, SPH/main.py:133-144).  The
+    reference rotates about trimesh's center_mass of a face-less mesh of the
+    points; the centroid is taken here (that point is not pinned by any
+    reference output)."""
     pts = np.asarray(points, dtype=np.float64)
     ang, axis = cfg.get("rotationAngle"), cfg.get("rotationAxis")
     if ang is not None and axis is not None:
@@ -127,14 +130,21 @@ def transform(points: np.ndarray, cfg: dict) -> np.ndarray:
 
 
 def fill_tank(height: float) -> np.ndarray:
-    """Lattice of pitch 2r filling [domain_start + pad, domain_end - pad] up to
-    `height` (fill_tank, SPH/main.py:147-181, voxelized with trimesh there)."""
+    """The particles filling the box [domain_start + pad, domain_end - pad] up
+    to `height` (fill_tank, SPH/main.py:147-181).  The reference voxelizes the
+    box mesh at pitch 2r with trimesh and fills it (points_from_volume,
+    :115-118: mesh.voxelized(pitch).fill().points); trimesh's subdivision
+    voxelizer rounds surface points to the nearest multiple of the pitch, so
+    the filled voxel centres are the multiples k * pitch per axis with k from
+    round(lo / pitch) to round(hi / pitch), x slowest and z fastest."""
     p = meta.parm
-    lo = np.asarray(p.domain_start) + p.padding
-    hi = np.asarray(p.domain_end) - p.padding
+    lo = np.asarray(p.domain_start, dtype=np.float64) + p.padding
+    hi = np.asarray(p.domain_end, dtype=np.float64) - p.padding
     hi = np.array([hi[0], height, hi[2]])
     pitch = p.particle_diameter
-    axes = [np.arange(lo[k], hi[k] + 0.5 * pitch, pitch) for k in range(3)]
+    k0 = np.round(lo / pitch).astype(np.int64)
+    k1 = np.round(hi / pitch).astype(np.int64)
+    axes = [np.arange(k0[k], k1[k] + 1, dtype=np.float64) * pitch for k in range(3)]
     g = np.meshgrid(*axes, indexing="ij")
     return np.stack([a.ravel() for a in g], axis=1)
 
@@ -497,7 +507,10 @@ def initialize(scene_path: str = "", scene=None, params: Optional[SceneParams] =
         initialize_particles(meta.pd)
         meta.export_ply = bool(get_cfg("exportPly", False))
     ctx = _lib.Context(meta.scene_params, meta.particle_max_num)
-    ctx.upload(meta.pd.x.to_numpy(), meta.pd.kind, meta.pd.v.to_numpy())
+    x0, v0 = meta.pd.x.to_numpy(), meta.pd.v.to_numpy()
+    # what init builds the rest state and static volumes from (snapshot/resume)
+    meta.initial_state = (np.array(x0, np.float32), np.array(v0, np.float32), np.array(meta.pd.kind, np.int32))
+    ctx.upload(x0, meta.pd.kind, v0)
     meta.ctx = ctx
     meta.ns = NeighborhoodSearchSparse(meta.pd.x, meta.particle_max_num, meta.parm.support_radius,
                                       meta.parm.domain_size, use_sparse_grid=True)
@@ -508,6 +521,43 @@ def initialize(scene_path: str = "", scene=None, params: Optional[SceneParams] =
     meta.elastic_bodies = [Elasticity(meta.particle_max_num)] if n_el else []
     meta.num_elastic_bodies = len(meta.elastic_bodies)
     meta.step_num = 0
+
+
+def snapshot(path: str) -> None:
+    """Write what a run continues from to `path` (.npz): the scene parameters,
+    the initial particle arrays (the rest state, the rest lists and the static
+    boundary volumes are rebuilt from them by init, as in the original run), the
+    current positions, velocities and Weiler warm-start direction, and the step
+    count.  The reference has no snapshot; it writes per-step PLY positions
+    (SPH/main.py:939-940), which carry no velocities."""
+    import dataclasses
+
+    x0, v0, kind = meta.initial_state
+    np.savez(path, params=np.array(json.dumps(dataclasses.asdict(meta.scene_params))), x0=x0, v0=v0, kind=kind,
+             x=meta.pd.x.to_numpy(), v=meta.pd.v.to_numpy(), vdiff=meta.ctx.download(_lib.FIELD_VISC_DIFF),
+             step_num=np.array(getattr(meta, "step_num", 0)))
+
+
+def resume(path: str):
+    """Continue a run from snapshot(path): a new context is initialized on the
+    snapshot's initial arrays, then given its positions, velocities and Weiler
+    warm-start direction.  Returns the solver; its steps equal, bit for bit,
+    those the original run would have taken."""
+    from .scenes import Scene
+
+    with np.load(path, allow_pickle=False) as z:
+        fields = {k: z[k] for k in z.files}
+    d = json.loads(str(fields["params"]))
+    params = SceneParams(**{k: tuple(v) if isinstance(v, list) else v for k, v in d.items()})
+    sc = Scene(name="resumed", params=params, x=fields["x0"], v=fields["v0"], kind=fields["kind"])
+    initialize(scene=sc)
+    solver = build_solver()
+    solver.initialize()
+    meta.ctx.upload_field(_lib.FIELD_X, fields["x"])
+    meta.ctx.upload_field(_lib.FIELD_V, fields["v"])
+    meta.ctx.upload_field(_lib.FIELD_VISC_DIFF, fields["vdiff"])
+    meta.step_num = int(fields["step_num"])
+    return solver
 
 
 def main(scene_path: str, max_step: Optional[int] = None):

@@ -86,3 +86,66 @@ def test_rigid_coupling_scene_json_run(tmp_path):
         orc.step()
     assert np.array_equal(meta.pd.x.to_numpy().astype(np.float64), orc.get(O.F_X))
     assert np.array_equal(meta.pd.v.to_numpy().astype(np.float64), orc.get(O.F_V))
+
+
+def test_main_exports_ply_every_step(tmp_path, monkeypatch):
+    """main() with exportPly writes results/<step>.ply after every step
+    (SPH/main.py:939-940, binary little-endian, vertex x y z float): each file
+    reads back as the positions of that step, here a tank of fluid filled by
+    FluidTanks (fill_tank, :147-181) beside a PLY block."""
+    root = tmp_path / "SPH"
+    (root / "data" / "models").mkdir(parents=True)
+    (root / "data" / "scenes").mkdir(parents=True)
+    block = scenes.lattice((0.3, 0.3, 0.3), (5, 5, 5))
+    ply.export_ply_particles(block, str(root / "data" / "models" / "block.ply"))
+    cfg = {"Configuration": {"domainStart": [0.0, 0.0, 0.0], "domainEnd": [0.6, 0.6, 0.6], "particleRadius": 0.01,
+                             "density0": 1000, "simulationMethod": 4, "gravitation": [0.0, -9.81, 0.0],
+                             "timeStepSize": 0.001, "exportPly": True, "maxStep": 3},
+           "FluidParticles": [{"geometryFile": "./data/models/block.ply", "translation": [0.0, 0.1, 0.0]}],
+           "FluidTanks": [{"height": 0.1}]}
+    scene = root / "data" / "scenes" / "tank.json"
+    scene.write_text(json.dumps(cfg))
+    monkeypatch.chdir(tmp_path)
+    solver = sph.main(str(scene))
+    n = meta.particle_max_num
+    assert n == 125 + 27 * 4 * 27  # block + tank: x, z multiples of 0.02 in [0.04, 0.56], y in [0.04, 0.10]
+    files = sorted(os.listdir(tmp_path / "results"))
+    assert files == ["0.ply", "1.ply", "2.ply"]
+    with open(tmp_path / "results" / "2.ply", "rb") as f:
+        head = f.read(200)
+    assert head.startswith(b"ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n)
+    last = ply.read_ply_particles(str(tmp_path / "results" / "2.ply"))
+    assert np.array_equal(last.astype(np.float32), meta.pd.x.to_numpy())
+    # the first file: the positions after one step of the same scene
+    sph.initialize(str(scene))
+    s2 = sph.build_solver()
+    s2.initialize()
+    meta.export_ply = False
+    s2.step()
+    first = ply.read_ply_particles(str(tmp_path / "results" / "0.ply"))
+    assert np.array_equal(first.astype(np.float32), meta.pd.x.to_numpy())
+
+
+def test_snapshot_and_resume_equal_the_uninterrupted_run(tmp_path):
+    """snapshot after 3 steps, 4 more steps; resume from the snapshot in a new
+    context, 4 steps: positions and velocities equal bit for bit (a fluid with
+    the Cross model and the warm-started Weiler CG, and an elastic block)."""
+    rng = np.random.Generator(np.random.PCG64(17))
+    sc = scenes.mixed_scene(rng, 3000, 2, 20.0, 1.0, True, "snap", dt=5e-4)
+    sph.initialize(scene=sc)
+    solver = sph.build_solver()
+    solver.initialize()
+    for _ in range(3):
+        solver.step()
+    path = str(tmp_path / "snap.npz")
+    sph.snapshot(path)
+    for _ in range(4):
+        solver.step()
+    x, v = meta.pd.x.to_numpy(), meta.pd.v.to_numpy()
+    assert meta.step_num == 7
+    solver = sph.resume(path)
+    assert meta.step_num == 3
+    for _ in range(4):
+        solver.step()
+    assert np.array_equal(meta.pd.x.to_numpy(), x)
+    assert np.array_equal(meta.pd.v.to_numpy(), v)

@@ -728,7 +728,7 @@ int KernelTimer::begin(int id, hipStream_t st) {
   open = true;
   while (events.size() < used + 2) {
     hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventCreateWithFlags(&e, TIMING_EVENT_FLAGS));
     events.push_back(e);
   }
   pending.push_back({id, used});

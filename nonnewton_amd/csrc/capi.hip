@@ -273,8 +273,8 @@ int32_t nn_event_overhead(int32_t reps, double* ms) {
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreate(&e0);
-  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&e0, nn::TIMING_EVENT_FLAGS);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&e1, nn::TIMING_EVENT_FLAGS);
   double total = 0.0;
   for (int r = -8; r < reps && e == hipSuccess; r++) {  // 8 untimed rounds first
     hipLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, st, 0);

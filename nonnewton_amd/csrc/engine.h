@@ -208,6 +208,11 @@ struct LaunchEvents {
   bool launched = false;
 };
 LaunchEvents& launch_events();  // of the calling thread
+// Flags of the timing events.  They only stamp times: without the
+// system-scope fence an event record would otherwise add (an L2 write-back and
+// invalidate on every XCD), a timed launch perturbs the launches around it less
+// (measured: about 2 us less idle per timed launch, 1% of a pooled tick).
+constexpr unsigned TIMING_EVENT_FLAGS = hipEventDisableSystemFence;
 
 #define NN_LAUNCH(kern, grid, blk, shm, st, ...)                                                   \
   do {                                                                                          \

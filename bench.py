@@ -5,21 +5,25 @@ counts log-uniform in [2k, 60k] (seed 3), viscosity model per scene from
 {Newtonian, Cross, power law}, eta0 log-uniform in [0.1, 100] solved by the
 Weiler implicit solver, 25% of scenes with an elastic block, dt 5e-4.
 
-  value     particle-steps/s of the batched form: all 64 scenes resident in one
-            pool, one launch sequence per tick (a particle-step = one dynamic
-            particle advanced one step; boundary particles are not counted)
+  value     particle-steps/s of the batched form: all 64 scenes resident,
+            spread over --lanes pools ("lanes", default 3) that are stepped
+            concurrently, each by its own host thread on its own stream with
+            one launch sequence per tick (a particle-step = one dynamic
+            particle advanced one step; boundary particles are not counted);
+            the same scenes in one pool are timed too (one_lane_*)
   plain     the same scenes stepped one at a time, one context each, over the
             same step indices as the batched form (same warm-up, same count)
   control   one scene pooled against the same scene in a context (about 1)
-  latency   p50/p99 scene-step latency: the tick time for the batched form,
-            the per-scene step time for the plain form
+  latency   p50/p99 scene-step latency: the lanes' tick times for the batched
+            form, the per-scene step time for the plain form
   roofline  the kernel with the largest share of a probe tick (on a separate
             pool): algorithmic bytes per launch (sweep table below, from the
             logged per-scene counts and iteration numbers) over its mean
             launch duration, from HIP events carried by every 8th launch of it
-            in the timed region, less the launch and event latency an empty
-            kernel shows when bracketed the same way; also a whole CG
-            iteration and the whole tick
+            in the one-lane form's timed region, less the launch and event
+            latency an empty kernel shows when bracketed the same way; also a
+            whole CG iteration, and the whole tick of the lanes and of the
+            one-lane form
   serving   continuous batching of the Poisson request stream (1k-500k
             particles, 50-500 steps) at 5, 20 and 80 requests/s
   cpu       the oracle (FP32, the reference's arithmetic, OpenMP over particles)
@@ -27,7 +31,8 @@ Weiler implicit solver, 25% of scenes with an elastic block, dt 5e-4.
             oracle as the checker: neighbor lists and fields of two scenes,
             the dam break's density-error drift against the FP64 oracle
 
-The timed ticks are bracketed by roctxProfilerResume/Pause, so
+The one-lane form's timed ticks (or, with --trace-region lanes, the lanes')
+are bracketed by roctxProfilerResume/Pause, so
 `rocprofv3 --selected-regions --kernel-trace --stats -- python bench.py ...`
 profiles exactly them.
 
@@ -293,6 +298,56 @@ def run_batched(scs, steps, warmup, profile=True, block=0, tune=True, kernel=Non
                 breakdown=breakdown, shapes=shapes, launches=launches)
 
 
+def run_lanes(scs, steps, warmup, lanes, block=0, shapes=None, region=None):
+    """The batched form in `lanes` lanes (LanePool): the scenes admitted largest
+    first, each to the least loaded lane (particles), every lane stepped by its
+    own host thread on its own stream with no barrier between lanes.  `warmup`
+    ticks of every lane, then `steps` timed ticks of every lane: the timed
+    region ends when the last lane has done its `steps` ticks, so every scene
+    advances exactly `steps` steps in it, the same step indices as the plain
+    form."""
+    from nonnewton_amd.lanes import LanePool
+
+    lp = LanePool(200 << 30, lanes=lanes, shapes=shapes, block=block)
+    order = sorted(range(len(scs)), key=lambda k: (-scs[k].n, k))
+    keys = {}
+    for k in order:
+        s = scs[k]
+        key = lp.admit(s.params, s.x, s.kind, s.v, n_elastic=s.n_elastic)
+        if key is None:
+            raise RuntimeError("pool budget too small for the workload")
+        keys[k] = key
+    slots = [[keys[k][1] for k in sorted(keys) if keys[k][0] == l] for l in range(lanes)]
+    scene_of = [[k for k in sorted(keys) if keys[k][0] == l] for l in range(lanes)]
+    lp.run(warmup, [[] for _ in range(lanes)])
+    with (region or _NoRegion()):
+        elapsed, out = lp.run(steps, slots)
+    lp.close()
+    lat = [t for lane in out for t in lane[0]]
+    # per tick k: every scene's statistics in workload order
+    names = out[0][1][0].dtype.names
+    stats = []
+    for k in range(steps):
+        row = [None] * len(scs)
+        for l in range(lanes):
+            a = out[l][1][k]
+            for q, sk in enumerate(scene_of[l]):
+                row[sk] = {f: a[f][q].item() for f in names}
+        stats.append(row)
+    return dict(elapsed=elapsed, lat=lat, stats=stats, lanes=lanes,
+                scenes_per_lane=[len(x) for x in scene_of],
+                particles_per_lane=[int(sum(scs[k].n for k in x)) for x in scene_of],
+                lane_ms_per_tick=[round(1e3 * sum(o[0]) / steps, 3) for o in out])
+
+
+class _NoRegion:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
 def run_plain(scs, steps, warmup, block=0, shapes=None):
     ctxs = []
     for s in scs:
@@ -346,17 +401,18 @@ def one_scene_control(sc, steps, warmup, shapes):
             "pooled_over_plain": round(plain / pooled, 3)}
 
 
-def run_serving(count, rate):
+def run_serving(count, rate, lanes=1, shapes=None):
     """Continuous batching over a Poisson request stream (sizes log-uniform 1k-500k,
-    50-500 steps, seed 4), 200 GB budget."""
+    50-500 steps, seed 4), 200 GB budget, in `lanes` lanes."""
     from nonnewton_amd.serving import ServingLoop
 
     reqs = S.serving_requests(seed=4, rate=rate, count=count)
-    loop = ServingLoop(200 << 30)
+    loop = ServingLoop(200 << 30, lanes=lanes, shapes=shapes)
     rep = loop.run(reqs)
     loop.close()
     s = rep.summary()
     s["rate_per_s"] = rate
+    s["lanes"] = lanes
     s["failed"] = int(sum(r.failed for r in rep.records))
     s["dynamic_particles_requested"] = int(sum(q.scene.n_dynamic for q in reqs))
     s["steps_requested"] = int(sum(q.steps for q in reqs))
@@ -574,6 +630,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-control", action="store_true", help="skip the one-scene pooled/plain control")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=3,
+                    help="lanes of the batched form (pools stepped concurrently; 1 = one pool)")
+    ap.add_argument("--trace-region", choices=("single", "lanes"), default="single",
+                    help="which timed ticks roctxProfilerResume/Pause bracket: the one-lane form's (whose "
+                         "launches carry the roofline's events) or the lanes'")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--serve-requests", type=int, default=64, help="requests per arrival rate (0 = no serving)")
     ap.add_argument("--serve-rates", default="5,20,80", help="Poisson arrival rates, requests/s")
@@ -604,9 +665,24 @@ def main():
     n_all = sum(s.n for s in scs)
     log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
 
+    lanes = max(1, args.lanes) if len(scs) > 1 else 1
     b = run_batched(scs, args.steps, args.warmup, block=args.block, tune=not args.no_tune,
-                    kernel=args.roofline_kernel)
-    elapsed, lat, stats, shapes = b["elapsed"], b["lat"], b["stats"], b["shapes"]
+                    kernel=args.roofline_kernel,
+                    region=TimedRegion() if (args.trace_region == "single" or lanes == 1) else _NoRegion())
+    shapes = b["shapes"]
+    # the single-lane form (one pool): its timed ticks carry the roofline's events
+    s_elapsed, s_stats = b["elapsed"], b["stats"]
+    s_value = n_dyn * args.steps / s_elapsed
+    log(f"[bench] batched, one lane: {1e3 * s_elapsed / args.steps:.3f} ms/tick, {s_value:.4g} particle-steps/s")
+    ln = None
+    if lanes > 1:
+        ln = run_lanes(scs, args.steps, args.warmup, lanes, block=args.block, shapes=shapes,
+                       region=TimedRegion() if args.trace_region == "lanes" else None)
+        elapsed, lat, stats = ln["elapsed"], ln["lat"], ln["stats"]
+        log(f"[bench] batched, {lanes} lanes: {1e3 * elapsed / args.steps:.3f} ms per tick of every lane "
+            f"(lanes {ln['lane_ms_per_tick']} ms/tick, scenes {ln['scenes_per_lane']})")
+    else:
+        elapsed, lat, stats = s_elapsed, b["lat"], s_stats
     value = n_dyn * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] batched: {ms_per_step:.3f} ms/tick, {value:.4g} particle-steps/s")
@@ -633,7 +709,7 @@ def main():
     extra = {}
     times = b["times"]
     if times:
-        kb = {k: sum(kernel_algorithmic_bytes(k, sc, st) or 0 for ss in stats for sc, st in zip(scs, ss))
+        kb = {k: sum(kernel_algorithmic_bytes(k, sc, st) or 0 for ss in s_stats for sc, st in zip(scs, ss))
               for k in times}
         # a timed launch's interval holds the launch and event latency as well
         # as the kernel: measured on an empty kernel bracketed the same way
@@ -642,7 +718,7 @@ def main():
         avg = {k: max(v - ovh, 1e-9) for k, v in raw.items()}
         name = b["dominant"]
         if name in avg and kernel_algorithmic_bytes(name, scs[0], stats[0][0]) is not None:
-            roof = roofline_entry(name, kb[name], avg[name], times[name][2], elapsed, args.workload)
+            roof = roofline_entry(name, kb[name], avg[name], times[name][2], s_elapsed, args.workload)
             roof["timed_launches"] = times[name][1]
             roof["event_interval_us"] = round(1e6 * raw[name], 2)
             roof["event_overhead_us"] = round(1e6 * ovh, 2)
@@ -655,11 +731,16 @@ def main():
                                      "ms_per_tick": round(1e3 * t_it / args.steps, 3)}
             if name != "cg_update" and "cg_update" in avg:
                 extra["cg_update"] = roofline_entry("cg_update", kb["cg_update"], avg["cg_update"],
-                                                    times["cg_update"][2], elapsed, args.workload)
-    tick_bytes = sum(step_algorithmic_bytes(sc, st) for ss in stats for sc, st in zip(scs, ss))
-    extra["tick"] = {"achieved": round(tick_bytes / elapsed / 1e9, 1),
-                     "frac": round(tick_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4),
-                     "bytes_per_tick": round(tick_bytes / args.steps)}
+                                                    times["cg_update"][2], s_elapsed, args.workload)
+    # the whole tick: every sweep of every scene over the timed region, for the
+    # line's form (lanes) and for the one-lane form the kernel figures come from
+    for key, st_, el_ in (("tick", stats, elapsed), ("tick_single_lane", s_stats, s_elapsed)):
+        tick_bytes = sum(step_algorithmic_bytes(sc, st) for ss in st_ for sc, st in zip(scs, ss))
+        extra[key] = {"achieved": round(tick_bytes / el_ / 1e9, 1),
+                      "frac": round(tick_bytes / el_ / 1e9 / HBM_PEAK_GBS, 4),
+                      "bytes_per_tick": round(tick_bytes / args.steps)}
+        if lanes == 1:
+            break
     if roof is not None:
         roof.update(extra)
     log(f"[bench] roofline: {json.dumps(roof)}")
@@ -673,6 +754,7 @@ def main():
         "scenes": len(scs),
         "particles": n_all,
         "dynamic_particles": n_dyn,
+        "form": f"{lanes} lanes" if lanes > 1 else "one pool",
         "batched_ms_per_tick": round(ms_per_step, 3),
         "p50_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 50)), 3),
         "p99_scene_step_ms_batched": round(1e3 * float(np.percentile(lat, 99)), 3),
@@ -684,6 +766,11 @@ def main():
         "non_finite_state": bool(nan),
         "launch_shapes": shapes,
     }
+    if ln is not None:
+        cfg["lanes"] = {"lanes": lanes, "scenes": ln["scenes_per_lane"], "particles": ln["particles_per_lane"],
+                        "ms_per_tick": ln["lane_ms_per_tick"]}
+        cfg["one_lane_ms_per_tick"] = round(1e3 * s_elapsed / args.steps, 3)
+        cfg["one_lane_particle_steps_per_s"] = round(s_value, 1)
     if plain_value is not None:
         cfg["plain_particle_steps_per_s"] = round(plain_value, 1)
         cfg["batched_over_plain"] = round(value / plain_value, 3)
@@ -699,7 +786,7 @@ def main():
     if args.serve_requests > 0 and args.workload == "mixed_batch":
         cfg["serving"] = []
         for rate in [float(r) for r in args.serve_rates.split(",") if r]:
-            s = run_serving(args.serve_requests, rate)
+            s = run_serving(args.serve_requests, rate, lanes=lanes, shapes=shapes)
             cfg["serving"].append(s)
             log(f"[bench] serving at {rate:g}/s: {s}")
 

@@ -209,7 +209,8 @@ int32_t nn_pool_destroy(nn_pool *pool);
 /* Bytes of device memory a scene of n particles (n_elastic elastic) occupies. */
 int64_t nn_pool_footprint(int64_t n, int64_t n_elastic, const nn_params *params);
 /* Admit a scene: uploaded and initialized now (on a side stream, overlapping a
- * running tick), joins the batch at the next tick.  NN_E_BUSY if over budget. */
+ * running tick), joins the batch at the next tick.  NN_E_BUSY if over budget
+ * (the bytes of scenes retired since the last tick count as free). */
 int32_t nn_pool_admit(nn_pool *pool, const nn_params *params, const float *x, const float *v,
                       const int32_t *kind, int64_t n, int32_t *slot);
 /* One step of every resident scene; per-slot stats into stats[slot] when given

@@ -142,7 +142,8 @@ class Batch {
   int n_cg_blocks_ = 0;  // entries of d.cg_blocks
   std::vector<void*> deferred_;
   int scan_blocks_ = 0;
-  SceneState* h_state_ = nullptr;  // pinned
+  SceneState* h_state_ = nullptr;  // pinned, h_state_cap_ entries
+  int h_state_cap_ = 0;
   int* h_active_ = nullptr;        // pinned
   bool any_fluid_ = false, any_elastic_ = false, any_weiler_ = false, any_strain_ = false, any_div_ = false;
   bool any_warm_cg_ = false, any_rigid_ = false;
@@ -151,8 +152,7 @@ class Batch {
   int rigid_step();
   int max_iter_[3] = {1, 1, 1};
 
-  template <class T>
-  int alloc(T*& p, size_t count);
+  int alloc_arena(std::vector<std::pair<void**, size_t>>& req);
   void release();
   int search_impl(int all = 1);
   int complete_lists();

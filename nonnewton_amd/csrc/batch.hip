@@ -235,20 +235,28 @@ Batch::~Batch() {
   if (h_active_) (void)hipHostFree(h_active_);
 }
 
-template <class T>
-int Batch::alloc(T*& p, size_t count) {
-  p = nullptr;
-  if (count == 0) count = 1;
+// The arrays of a layout are carved from one device allocation (256-byte
+// aligned sub-ranges): one hipMalloc and, when the batch is released or laid
+// out again, one hipFree, which synchronizes the device -- a pool admission
+// creates and later frees a staging batch, and with one allocation per array
+// it paid that synchronization some forty times.
+int Batch::alloc_arena(std::vector<std::pair<void**, size_t>>& req) {
+  size_t total = 0;
+  for (auto& r : req) total += (r.second + 255) & ~(size_t)255;
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  hipError_t e = hipMalloc(&q, total);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
     return NN_E_OOM;
   }
-  allocs_.push_back({q, count * sizeof(T)});
-  bytes_ += (long long)(count * sizeof(T));
-  p = (T*)q;
+  char* c = (char*)q;
+  for (auto& r : req) {
+    *r.first = c;
+    c += (r.second + 255) & ~(size_t)255;
+  }
+  allocs_.push_back({q, total});
+  bytes_ += (long long)total;
   return NN_OK;
 }
 
@@ -461,20 +469,18 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   nd.nscenes = (int)scenes.size();
   const int ncell_entries = Cc + 1;
   const int nscan = (ncell_entries + 2047) / 2048;
-  int rc = NN_OK;
+  std::vector<std::pair<void**, size_t>> req;
   auto A = [&](auto*& p, size_t count) {
-    if (rc == NN_OK) rc = alloc(p, count);
+    using T = std::remove_reference_t<decltype(*p)>;
+    req.push_back({(void**)&p, (count ? count : 1) * sizeof(T)});
   };
-  SceneDev* scenes_dev;
+  SceneDev* scenes_dev = nullptr;
   A(scenes_dev, Sc);
-  nd.scenes = scenes_dev;
   A(nd.state, Sc);
-  int* ws;
+  int* ws = nullptr;
   A(ws, nd.W);
-  nd.wave_scene = ws;
-  int* sws;
+  int* sws = nullptr;
   A(sws, nd.SW);
-  nd.s_wave_scene = sws;
   A(nd.pos, Pc);
   A(nd.pos_alt, Pc);
   A(nd.vel, Pc);
@@ -494,9 +500,8 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.cell_start, ncell_entries);
   A(nd.scan_tmp, nscan);
   A(nd.partials, 32 * (size_t)nd.W);
-  int2* cgb;
+  int2* cgb = nullptr;
   A(cgb, (size_t)nd.W / 16 + Sc + 1);
-  nd.cg_blocks = cgb;
   A(nd.cg_list, 2 * (size_t)nd.W);
   A(nd.cg_list_n, 2);
   A(nd.n_active, 4);
@@ -520,20 +525,25 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.rest, (size_t)NSc * MAX_REST);
   A(nd.rest_cnt, NSc);
   A(nd.rs, 4 * (size_t)NSc);
-  int* rpid;
+  int* rpid = nullptr;
   A(rpid, NRc);
-  nd.r_pid = rpid;
-  int* rwb;
+  int* rwb = nullptr;
   A(rwb, NRc / 64);
-  nd.r_wave_body = rwb;
   A(nd.rpartials, 9 * (size_t)(NRc / 64));
   A(nd.bodies, NBc);
+  const int rc = alloc_arena(req);
   if (rc != NN_OK) {
     release();
     allocs_.swap(old_allocs);
     bytes_ = old_bytes;
     return rc;
   }
+  nd.scenes = scenes_dev;
+  nd.wave_scene = ws;
+  nd.s_wave_scene = sws;
+  nd.cg_blocks = cgb;
+  nd.r_pid = rpid;
+  nd.r_wave_body = rwb;
   for (size_t k = 0; k < scenes.size(); k++) {
     scenes[k]->p_begin = pb[k];
     scenes[k]->cell_begin = cb[k];
@@ -547,10 +557,15 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   NB_cap_ = NBc;
   scene_cap_ = Sc;
   scan_blocks_ = nscan;
-  if (!h_state_) {
-    HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * 4096, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
+  // pinned host copies of the scene states, for as many scenes as the layout
+  // holds (a staging batch holds one; hipHostFree synchronizes the device)
+  if (!h_state_ || h_state_cap_ < Sc) {
+    if (h_state_) HIPCHK(hipHostFree(h_state_));
+    h_state_ = nullptr;
+    HIPCHK(hipHostMalloc((void**)&h_state_, sizeof(SceneState) * (size_t)std::max(Sc, 1), hipHostMallocDefault));
+    h_state_cap_ = std::max(Sc, 1);
   }
+  if (!h_active_) HIPCHK(hipHostMalloc((void**)&h_active_, sizeof(int) * 4, hipHostMallocDefault));
   CHK(upload_tables(nd));
   HIPCHK(hipMemsetAsync(nd.state, 0, sizeof(SceneState) * std::max<size_t>(Sc, 1), stream));
   launch_fill_i32(nd.info, Pc, -1, stream);

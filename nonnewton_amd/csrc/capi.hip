@@ -347,11 +347,36 @@ static constexpr int POOL_CAPACITY = 4096;
 struct PoolSlot {
   std::shared_ptr<SceneHost> scene;
   std::unique_ptr<Batch> staging;  // set while the scene waits to join the batch
-  hipStream_t staging_stream = nullptr;
   long long bytes = 0;
   long long steps = 0;  // ticks this scene took part in
   bool used = false, retired = false;
 };
+
+// One staging stream per device for every pool's admissions (upload and
+// initialization of a new scene), created at the first admission and kept for
+// the life of the process.  The device maps streams onto a few hardware
+// queues in creation order; pools create their tick streams first, so this
+// one does not queue behind a pool's ticks, as a stream per admission slot
+// would once there are more streams than hardware queues.
+static int staging_stream(hipStream_t* out) {
+  static std::mutex mu;
+  static std::vector<std::pair<int, hipStream_t>> streams;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nn::hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& ds : streams)
+    if (ds.first == dev) {
+      *out = ds.second;
+      return NN_OK;
+    }
+  hipStream_t st = nullptr;
+  e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) return nn::hip_fail(e, "hipStreamCreate");
+  streams.push_back({dev, st});
+  *out = st;
+  return NN_OK;
+}
 
 struct nn_pool {
   hipStream_t stream = nullptr;
@@ -438,7 +463,6 @@ int32_t nn_pool_destroy(nn_pool* p) {
   p->main.reset();
   for (auto& s : p->slots) {
     s.staging.reset();
-    if (s.staging_stream) (void)hipStreamDestroy(s.staging_stream);
   }
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -477,7 +501,12 @@ int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const
   int s_id = -1;
   {
     std::lock_guard<std::mutex> lk(p->mu);
-    if (p->in_use + 2 * fp > p->budget) {  // the scene lives twice while it is merged
+    // scenes retired since the last tick leave at the next membership change:
+    // their bytes are already free for the budget
+    long long retiring = 0;
+    for (const PoolSlot& q : p->slots)
+      if (q.used && q.retired) retiring += q.bytes;
+    if (p->in_use - retiring + 2 * fp > p->budget) {  // the scene lives twice while it is merged
       nn::set_error("pool budget exceeded");
       return NN_E_BUSY;
     }
@@ -505,11 +534,12 @@ int32_t nn_pool_admit(nn_pool* p, const nn_params* params, const float* x, const
     p->in_use -= fp;
     return code;
   };
-  if (!ps.staging_stream) {
-    hipError_t e = hipStreamCreateWithFlags(&ps.staging_stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(nn::hip_fail(e, "hipStreamCreate"));
+  hipStream_t sst = nullptr;
+  {
+    const int rc2 = staging_stream(&sst);
+    if (rc2 < 0) return fail(rc2);
   }
-  std::unique_ptr<Batch> stg(new Batch(ps.staging_stream));
+  std::unique_ptr<Batch> stg(new Batch(sst));
   stg->block = p->block;
   std::copy(p->kblock, p->kblock + nn::K_COUNT, stg->kblock);
   std::copy(p->kdepth, p->kdepth + nn::K_COUNT, stg->kdepth);

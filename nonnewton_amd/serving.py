@@ -1,11 +1,11 @@
 """Serving loop: continuous batching of independent SPH scenes on one GPU.
 
 Requests (a scene and a number of steps) arrive over time.  An admission thread
-uploads and initializes each arrived request into the pool as soon as the memory
+uploads and initializes each arrived request into a pool as soon as the memory
 budget allows (nn_pool_admit runs on its own HIP stream, so it overlaps the tick
-running on the pool's stream); the main thread ticks the pool -- one step of
-every resident scene per tick -- and retires a scene when its steps are done,
-which frees budget for waiting requests.  A scene's results do not depend on
+running on the pool's stream); each lane's thread ticks its pool -- one step of
+every resident scene of that lane per tick -- and retires a scene when its steps
+are done, which frees budget for waiting requests.  A scene's results do not depend on
 which other scenes share its ticks (every sum is per scene and in a fixed
 order), so a request gives the same bits alone and in the mix.
 
@@ -17,11 +17,12 @@ from __future__ import annotations
 import threading
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
 from . import _lib
+from .lanes import Key, LanePool
 
 
 @dataclass
@@ -32,7 +33,7 @@ class RequestRecord:
     n_dynamic: int
     admitted: Optional[float] = None
     finished: Optional[float] = None
-    slot: Optional[int] = None
+    slot: Optional[Tuple[int, int]] = None  # (lane, slot of that lane's pool)
     done_steps: int = 0
     failed: bool = False
     error: str = ""  # why it failed: rejected at admission or evicted by a step's status
@@ -72,19 +73,28 @@ class ServingReport:
 
 
 class ServingLoop:
-    """Continuous batching over an nn_pool with admission by memory budget."""
+    """Continuous batching over lanes of nn_pools with admission by memory budget.
 
-    def __init__(self, budget_bytes: int = 200 << 30, block: int = 0, keep_results: bool = False):
-        self.pool = _lib.Pool(budget_bytes)
+    lanes = 1: one pool, ticked by one thread.  lanes = K: the scenes spread over
+    K pools (nonnewton_amd.lanes.LanePool), each ticked by its own thread on its
+    own stream with no barrier between them; an arriving request goes to the
+    least loaded lane.  The budget holds over all lanes together."""
+
+    def __init__(self, budget_bytes: int = 200 << 30, block: int = 0, keep_results: bool = False, lanes: int = 1,
+                 shapes=None):
+        self.lanes = LanePool(budget_bytes, lanes=lanes, shapes=shapes, block=block)
         self.budget = int(budget_bytes)
-        if block:
-            self.pool.set_block(block)
         self.keep_results = keep_results
         self._lock = threading.Lock()
         self._cond = threading.Condition(self._lock)
 
+    @property
+    def pool(self) -> _lib.Pool:
+        """The first lane's pool (the only one with lanes = 1)."""
+        return self.lanes.pools[0]
+
     def close(self):
-        self.pool.close()
+        self.lanes.close()
 
     def run(self, requests, time_scale: float = 1.0) -> ServingReport:
         """Serve `requests` (objects with rid, arrival [s], scene, steps).  Arrival
@@ -93,9 +103,15 @@ class ServingLoop:
         scenes = {q.rid: q.scene for q in requests}
         order = sorted(recs, key=lambda r: r.arrival)
         t0 = time.perf_counter()
-        resident: Dict[int, RequestRecord] = {}
+        lp = self.lanes
+        K = lp.lanes
+        resident: Dict[Key, RequestRecord] = {}
         pending: List[RequestRecord] = list(order)
         state = {"stop": False, "freed": 0}
+        acc = {"ticks": 0, "psteps": 0, "peak_res": 0, "peak_bytes": 0}
+        lane_res = [0] * K
+        lane_bytes = [0] * K
+        tick_ms: List[float] = []
 
         def reject_locked(rec, why):  # with self._cond held
             pending.remove(rec)
@@ -124,17 +140,17 @@ class ServingLoop:
                 sc = scenes[nxt.rid]
                 # a scene is held twice while it joins the batch: one that
                 # needs more than the whole budget that way can never be admitted
-                fp = self.pool.footprint(sc.params, sc.n, sc.n_elastic)
+                fp = lp.footprint(sc.params, sc.n, sc.n_elastic)
                 if fp < 0 or 2 * fp > self.budget:
                     reject(nxt, f"needs {2 * fp} bytes to join, over the budget of {self.budget}")
                     continue
                 try:
-                    slot = self.pool.admit(sc.params, sc.x, sc.kind, sc.v)
+                    key = lp.admit(sc.params, sc.x, sc.kind, sc.v, n_elastic=sc.n_elastic)
                 except _lib.NNError as e:  # out of grid, cell/neighbor overflow, bad kinds ...
                     reject(nxt, str(e))
                     continue
                 with self._cond:
-                    if slot is None:  # over budget: wait for a retirement
+                    if key is None:  # over budget: wait for a retirement
                         while state["freed"] == freed_seen and not state["stop"]:
                             if not resident:  # nothing will retire: the request cannot fit now
                                 break
@@ -143,80 +159,93 @@ class ServingLoop:
                             reject_locked(nxt, "over the budget with nothing resident")
                         continue
                     pending.pop(0)
-                    nxt.slot = slot
+                    nxt.slot = key
                     nxt.admitted = time.perf_counter() - t0
-                    resident[slot] = nxt
+                    resident[key] = nxt
                     self._cond.notify_all()
 
-        def admitter_guarded():
-            try:
-                admitter()
-            except BaseException as e:  # keep the main loop from waiting on a dead thread
-                with self._cond:
-                    state["admit_error"] = e
-                    self._cond.notify_all()
+        def guarded(fn, tag):
+            def body(*a):
+                try:
+                    fn(*a)
+                except BaseException as e:  # keep the other threads from waiting on a dead one
+                    with self._cond:
+                        state.setdefault("error", (tag, e))
+                        state["stop"] = True
+                        self._cond.notify_all()
+            return body
 
-        th = threading.Thread(target=admitter_guarded, daemon=True)
-        th.start()
-        tick_ms: List[float] = []
-        ticks = 0
-        psteps = 0
-        peak_res = 0
-        peak_bytes = 0
-        try:
+        adm = threading.Thread(target=guarded(admitter, "admission"), daemon=True)
+
+        def lane_loop(l: int):
+            pool = lp.pools[l]
             while True:
                 with self._cond:
-                    if "admit_error" in state:
-                        raise RuntimeError("admission thread failed") from state["admit_error"]
-                    active = {s: r for s, r in resident.items() if r.finished is None}
-                    if not active and not pending:
-                        break
+                    if state["stop"]:
+                        return
+                    active = [k for k, r in resident.items() if k[0] == l and r.finished is None]
                     if not active:
-                        if not th.is_alive() and pending:
+                        if not pending and not resident:
+                            return  # every request is done
+                        if pending and not adm.is_alive():
                             raise RuntimeError("admission thread ended with requests pending")
                         self._cond.wait(timeout=0.01)
                         continue
                 t = time.perf_counter()
-                rc, stepped = self.pool.tick()
+                pool.tick()
                 dt = time.perf_counter() - t
-                r, q, b = self.pool.info()
-                peak_res = max(peak_res, r)
-                peak_bytes = max(peak_bytes, b)
+                r, q, b = pool.info()
                 # a scene admitted during the tick joins at the next one: count
                 # what the pool actually stepped
-                with self._cond:
-                    live = [s for s, rec in resident.items() if rec.finished is None]
                 done = []
-                for s in live:
-                    rec = resident[s]
-                    n = self.pool.steps(s)
+                psteps = 0
+                for key in active:
+                    rec = resident[key]
+                    n = pool.steps(key[1])
                     if n > rec.done_steps:
                         psteps += (n - rec.done_steps) * rec.n_dynamic
                         rec.done_steps = n
-                        if self.pool.stats(s)["status"] & _lib.Pool.STATUS_ERROR_BITS:
+                        if pool.stats(key[1])["status"] & _lib.Pool.STATUS_ERROR_BITS:
                             rec.failed = True  # evicted; the other scenes continue unperturbed
-                            done.append(s)
+                            done.append(key)
                             continue
                     if rec.done_steps >= rec.steps:
-                        done.append(s)
-                tick_ms.append(1e3 * dt)
-                ticks += 1
-                for s in done:
-                    rec = resident[s]
+                        done.append(key)
+                with self._cond:
+                    tick_ms.append(1e3 * dt)
+                    acc["ticks"] += 1
+                    acc["psteps"] += psteps
+                    lane_res[l], lane_bytes[l] = r, b
+                    acc["peak_res"] = max(acc["peak_res"], sum(lane_res))
+                    acc["peak_bytes"] = max(acc["peak_bytes"], sum(lane_bytes))
+                for key in done:
+                    rec = resident[key]
                     if self.keep_results:
-                        rec.result = {"x": self.pool.read(s, _lib.FIELD_X), "v": self.pool.read(s, _lib.FIELD_V)}
+                        rec.result = {"x": pool.read(key[1], _lib.FIELD_X), "v": pool.read(key[1], _lib.FIELD_V)}
                     rec.finished = time.perf_counter() - t0
-                    self.pool.retire(s)
+                    lp.retire(key)
                     with self._cond:
-                        del resident[s]
+                        del resident[key]
                         state["freed"] += 1
                         self._cond.notify_all()
+
+        workers = [threading.Thread(target=guarded(lane_loop, f"lane {l}"), args=(l,), daemon=True) for l in range(K)]
+        adm.start()
+        for w in workers:
+            w.start()
+        try:
+            for w in workers:
+                w.join()
         finally:
             with self._cond:
                 state["stop"] = True
                 self._cond.notify_all()
-            th.join(timeout=10)
+            adm.join(timeout=10)
+        if "error" in state:
+            tag, e = state["error"]
+            raise RuntimeError(f"serving {tag} thread failed") from e
         wall = time.perf_counter() - t0
         lat = [r.finished - r.arrival for r in recs if r.finished is not None]
         wait = [r.admitted - r.arrival for r in recs if r.admitted is not None]
-        return ServingReport(wall, ticks, psteps, tick_ms, lat, wait, peak_res, peak_bytes, recs)
+        return ServingReport(wall, acc["ticks"], acc["psteps"], tick_ms, lat, wait, acc["peak_res"],
+                             acc["peak_bytes"], recs)

@@ -95,3 +95,18 @@ def test_serving_rejects_what_cannot_be_admitted():
     for k in (0, 3):
         assert not r[k].failed and r[k].done_steps == 2
         assert np.array_equal(r[k].result["x"], alone(reqs[k].scene, 2)["x"])
+
+
+def test_serving_in_lanes_equals_alone():
+    """Three lanes (pools on their own streams, each ticked by its own thread):
+    every request gives the bits it gives alone, and the lanes all take part."""
+    reqs = requests(steps=(3, 5, 2, 4, 3, 2), seed=7)
+    loop = ServingLoop(8 << 30, keep_results=True, lanes=3)
+    rep = loop.run(reqs, time_scale=0.0)  # all arrive at once, so they overlap
+    loop.close()
+    assert all(r.finished is not None and not r.failed for r in rep.records)
+    assert {r.slot[0] for r in rep.records} == {0, 1, 2}
+    for rec, q in zip(rep.records, reqs):
+        ref = alone(q.scene, q.steps)
+        assert rec.done_steps == q.steps
+        assert np.array_equal(rec.result["x"], ref["x"]) and np.array_equal(rec.result["v"], ref["v"])

@@ -205,6 +205,13 @@ int32_t nn_counts(nn_ctx *ctx, int64_t *n, int64_t *n_fluid);
  * particle array and advances all of them with one launch sequence per tick.
  * ------------------------------------------------------------------------- */
 int32_t nn_pool_create(int64_t budget_bytes, nn_pool **out);
+/* nn_pool_create with flags.  NN_POOL_OWN_QUEUE: the pool's stream gets a
+ * hardware queue of its own (a stream whose CU mask covers every CU), so that
+ * pools ticked concurrently from several threads (lanes) never share a queue;
+ * with plain streams the device maps streams onto a few shared hardware
+ * queues, and two pools on one queue serialize. */
+#define NN_POOL_OWN_QUEUE 1
+int32_t nn_pool_create_ex(int64_t budget_bytes, int32_t flags, nn_pool **out);
 int32_t nn_pool_destroy(nn_pool *pool);
 /* Bytes of device memory a scene of n particles (n_elastic elastic) occupies. */
 int64_t nn_pool_footprint(int64_t n, int64_t n_elastic, const nn_params *params);

@@ -33,6 +33,7 @@ NN_E_BUSY = -9
 NN_E_STATE = -10
 NN_E_REST_OVERFLOW = -11
 NN_W_NOCONV = 1
+POOL_OWN_QUEUE = 1  # NN_POOL_OWN_QUEUE
 
 ERROR_NAMES = {
     NN_E_ARG: "NN_E_ARG",
@@ -153,6 +154,7 @@ def load():
         "nn_counts": (i32, [vp, P(i64), P(i64)]),
         "nn_set_block": (i32, [vp, i32]),
         "nn_pool_create": (i32, [i64, P(vp)]),
+        "nn_pool_create_ex": (i32, [i64, i32, P(vp)]),
         "nn_pool_destroy": (i32, [vp]),
         "nn_pool_footprint": (i64, [i64, i64, P(NnParams)]),
         "nn_pool_admit": (i32, [vp, P(NnParams), P(ctypes.c_float), P(ctypes.c_float), P(i32), i64, P(i32)]),
@@ -368,10 +370,13 @@ class Context:
 class Pool:
     """Many resident scenes stepped together (nn_pool)."""
 
-    def __init__(self, budget_bytes: int):
+    def __init__(self, budget_bytes: int, own_queue: bool = False):
+        """own_queue: the pool's stream gets a hardware queue of its own
+        (NN_POOL_OWN_QUEUE), for pools stepped concurrently (lanes)."""
         self.lib = load()
         h = ctypes.c_void_p()
-        check(self.lib.nn_pool_create(int(budget_bytes), ctypes.byref(h)), "nn_pool_create")
+        check(self.lib.nn_pool_create_ex(int(budget_bytes), POOL_OWN_QUEUE if own_queue else 0, ctypes.byref(h)),
+              "nn_pool_create_ex")
         self.h = h
         self.capacity = int(self.lib.nn_pool_capacity())
         self._stats = (NnStepStats * self.capacity)()

@@ -435,11 +435,30 @@ int32_t nn_surface_run_context(nn_surface* s, nn_ctx* ctx) {
 
 int32_t nn_pool_capacity(void) { return POOL_CAPACITY; }
 
-int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) {
-  if (!out || budget_bytes <= 0) return NN_E_ARG;
+// A stream on a hardware queue of its own: a CU mask is a property of the
+// queue, so a stream created with one never shares its queue.  The mask is the
+// full one a plain stream reports (every CU of the device).
+static hipError_t own_queue_stream(hipStream_t* out) {
+  hipStream_t tmp = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&tmp, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  uint32_t mask[32] = {};
+  e = hipExtStreamGetCUMask(tmp, 32, mask);
+  (void)hipStreamDestroy(tmp);
+  if (e != hipSuccess) return e;
+  uint32_t words = 32;
+  while (words > 1 && mask[words - 1] == 0) words--;
+  return hipExtStreamCreateWithCUMask(out, words, mask);
+}
+
+int32_t nn_pool_create(int64_t budget_bytes, nn_pool** out) { return nn_pool_create_ex(budget_bytes, 0, out); }
+
+int32_t nn_pool_create_ex(int64_t budget_bytes, int32_t flags, nn_pool** out) {
+  if (!out || budget_bytes <= 0 || (flags & ~NN_POOL_OWN_QUEUE)) return NN_E_ARG;
   auto p = new (std::nothrow) nn_pool();
   if (!p) return NN_E_OOM;
-  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  hipError_t e = (flags & NN_POOL_OWN_QUEUE) ? own_queue_stream(&p->stream)
+                                             : hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete p;
     return nn::hip_fail(e, "hipStreamCreate");

@@ -12,9 +12,9 @@ every sum is per scene in a fixed order, so a scene gives the same bits in any
 lane and beside any other lanes.
 
 Scenes go to the lane with the smallest load (particles resident, or the cost
-the caller gives).  Streams are created with the lanes, before any admission,
-so that the lanes' streams are the first of the process to be mapped to the
-device's hardware queues.
+the caller gives).  With more than one lane each pool's stream has a hardware
+queue of its own (NN_POOL_OWN_QUEUE): the device maps plain streams onto a few
+shared queues, and two lanes on one queue would run one after the other.
 """
 from __future__ import annotations
 
@@ -37,8 +37,10 @@ class LanePool:
         if lanes < 1:
             raise ValueError("lanes must be at least 1")
         self.budget = int(budget_bytes)
-        # each lane may use the whole budget; admit() keeps the sum within it
-        self.pools = [_lib.Pool(self.budget) for _ in range(lanes)]
+        # each lane may use the whole budget; admit() keeps the sum within it.
+        # With several lanes every pool's stream gets a hardware queue of its
+        # own: lanes that shared one would run one after the other.
+        self.pools = [_lib.Pool(self.budget, own_queue=lanes > 1) for _ in range(lanes)]
         for p in self.pools:
             if block:
                 p.set_block(block)

@@ -382,15 +382,24 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
 constexpr int UPD_THREADS = 1024;
 constexpr int UPD_WAVES = UPD_THREADS / 64;
 
+// The scene's first workgroup may clear active_cg in this same launch, so the
+// flag is read once per workgroup (one thread, through LDS): a workgroup whose
+// waves read it on both sides of that store would split, some waves leaving
+// and the rest reducing with wave sums that the others never wrote.  A
+// workgroup that still sees the flag set reaches the first workgroup's
+// decision from the same sums and leaves without writing.
 __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
   __shared__ double red[3][UPD_WAVES];
+  __shared__ int active;
   const int2 e = d.cg_blocks[blockIdx.x];
   const int s = e.x;
   const SceneDev& sc = d.scenes[s];
   SceneState* st = d.state + s;
   const int t = threadIdx.x;
   if (blockIdx.x == 0 && t == 0) d.cg_list_n[(k + 1) & 1] = 0;  // for update k + 1
-  if (!st->active_cg) return;  // uniform over the workgroup
+  if (t == 0) active = st->active_cg;
+  __syncthreads();
+  if (!active) return;  // uniform over the workgroup
   const bool lead = e.y == sc.wave_begin && t == 0;
   double al, be;
   if (k == 1) {

@@ -61,6 +61,11 @@ def test_argument_errors_without_gpu():
     bad = L.params_struct(SceneParams(warm_start=2))
     assert lib.nn_create(ctypes.byref(bad), 10, ctypes.byref(h)) == L.NN_E_UNSUPPORTED
     assert lib.nn_pool_footprint(1000, 0, ctypes.byref(L.params_struct(SceneParams()))) > 1000 * 500
+    # pool flags: anything but NN_POOL_OWN_QUEUE, a zero budget or no output is refused before any HIP call
+    assert lib.nn_pool_create_ex(1 << 30, 2, ctypes.byref(h)) == L.NN_E_ARG
+    assert lib.nn_pool_create_ex(1 << 30, L.POOL_OWN_QUEUE | 4, ctypes.byref(h)) == L.NN_E_ARG
+    assert lib.nn_pool_create_ex(0, L.POOL_OWN_QUEUE, ctypes.byref(h)) == L.NN_E_ARG
+    assert lib.nn_pool_create_ex(1 << 30, 0, None) == L.NN_E_ARG
 
 
 def test_product_does_not_import_the_oracle():

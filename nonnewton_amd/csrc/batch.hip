@@ -327,7 +327,7 @@ int Batch::upload_tables(const Dev& nd) {
     hs[k] = s.dev;
     const int f = s.dev.flags;
     if ((f & SF_HAS_FLUID) && (f & SF_VISCOSITY) && (f & SF_WEILER))  // CG update workgroups
-      for (int w = s.dev.wave_begin; w < s.dev.wave_end; w += 16) cgb.push_back(make_int2((int)k, w));
+      for (int w = s.dev.wave_begin; w < s.dev.wave_end; w += CG_UPD_WAVES) cgb.push_back(make_int2((int)k, w));
   }
   n_cg_blocks_ = (int)cgb.size();
   // rigid set and body table, rebuilt from the scenes' host data (ranges of
@@ -501,7 +501,7 @@ int Batch::layout(const std::vector<const Batch*>& from, double slack, int scene
   A(nd.scan_tmp, nscan);
   A(nd.partials, 32 * (size_t)nd.W);
   int2* cgb = nullptr;
-  A(cgb, (size_t)nd.W / 16 + Sc + 1);
+  A(cgb, (size_t)nd.W / CG_UPD_WAVES + Sc + 1);
   A(nd.cg_list, 2 * (size_t)nd.W);
   A(nd.cg_list_n, 2);
   A(nd.n_active, 4);

@@ -32,6 +32,20 @@ constexpr int MAX_NB = 60;    // NeighborhoodSearchSparse.max_num_neighbors, SPH
 constexpr int MAX_PIC = 50;   // max_num_particles_in_grid, SPH/main.py:778
 constexpr int MAX_REST = 50;  // Elasticity.max_num_neighbors, SPH/main.py:1192
 constexpr int WAVE = 64;
+// Workgroup sizes of the per-scene reductions.  Smaller workgroups find room
+// on a CU sooner while other pools' sweeps occupy the device (lanes): with 8
+// waves per CG-update workgroup and 256 threads per scene finish the mixed
+// batch ticks in 3.11 ms in three lanes against 3.23 with 16 waves and 1,024
+// threads (3.54 against 3.59 in one pool), with the same bits.
+#ifndef NN_UPD_WAVES
+#define NN_UPD_WAVES 8
+#endif
+#ifndef NN_FINISH_THREADS
+#define NN_FINISH_THREADS 256
+#endif
+// waves of one CG-update workgroup (its cg_blocks entry covers that many
+// waves of one scene)
+constexpr int CG_UPD_WAVES = NN_UPD_WAVES;
 
 enum { KIND_FLUID = 0, KIND_STATIC = 1, KIND_ELASTIC = 2, KIND_RIGID = 3 };
 constexpr int MAX_BODIES = 4096;  // rigid bodies per scene
@@ -150,7 +164,7 @@ struct Dev {
   int* cell_start;
   int* scan_tmp;
   double* partials;  // 8 arrays of 4W: wave partials (per 64 particles, or per 16 for four-lane sweeps)
-  const int2* cg_blocks;  // CG update workgroups: (scene, first wave), up to 16 waves of one scene each
+  const int2* cg_blocks;  // CG update workgroups: (scene, first wave), up to CG_UPD_WAVES waves of one scene each
   int* cg_list;           // 2 x W: waves of the scenes still iterating, by iteration parity
   int* cg_list_n;         // 2 counts
   int* n_active;     // [0] divergence, [1] pressure, [2] cg

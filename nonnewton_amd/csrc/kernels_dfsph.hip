@@ -358,16 +358,17 @@ __global__ void k_press_source(Dev d, int reduce) {
   if (reduce) wave_partials<1>(d, (int)(i >> 6), v);
 }
 
-// Second stage of the per-scene reductions: one 1024-thread workgroup per scene
-// adds the partials of its waves and applies the loop test of the solver.
+// Second stage of the per-scene reductions: one FINISH_THREADS-thread workgroup
+// per scene adds the partials of its waves and applies the loop test of the
+// solver.
 //   which 0: divergence solve (:1695-1703), 1: pressure solve (:1778-1784),
 //         2: CG start (after cg_guess and the first matvec),
 //         4: scale of the initial guess (after cg_init, SF_WARM_CG scenes).
 // (The CG's per-iteration reduction is done inside k_cg_update.)
-// Thread t adds entries t, t + 1024, ... in order (two interleaved chains);
-// the sixteen wavefront sums are combined by a fixed pairwise tree: a fixed
+// Thread t adds entries t, t + FINISH_THREADS, ... in order (two interleaved
+// chains); the wavefront sums are combined by a fixed pairwise tree: a fixed
 // order, the same bits every run.
-constexpr int FINISH_THREADS = 1024;
+constexpr int FINISH_THREADS = NN_FINISH_THREADS;
 constexpr int FINISH_WAVES = FINISH_THREADS / 64;
 
 __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int which) {

@@ -369,7 +369,7 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
 
 // Iteration k of the CG with the scene's reduction of iteration k - 1 done in
 // the same launch (no separate reduction kernel per iteration).  Workgroup =
-// up to 16 waves of one scene (d.cg_blocks).  Every workgroup of a scene
+// up to CG_UPD_WAVES waves of one scene (d.cg_blocks).  Every workgroup of a scene
 // first adds the scene's partials -- r.r and r.u of update k - 1, u.w of
 // matvec k - 1 -- in one fixed order, so all of them obtain the same bits and
 // take the same decision:
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
 //     alpha_old) (iteration 1 takes alpha from the start reduction, beta = 0)
 // and then p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
 // u = M r, with partials (r.r, r.u) into arrays 4 + 2 (k & 1), 5 + 2 (k & 1).
-constexpr int UPD_THREADS = 1024;
+constexpr int UPD_THREADS = 64 * CG_UPD_WAVES;
 constexpr int UPD_WAVES = UPD_THREADS / 64;
 
 // The scene's first workgroup may clear active_cg in this same launch, so the

@@ -106,7 +106,10 @@ struct BodyDev {
 
 // Per-scene mutable solver state.
 struct SceneState {
-  int active_v, active_p, active_cg, pad_a;
+  // active_cg: the scene's CG runs in this solve (set by the start reduction);
+  // cg_stop_at: the update k that stopped it (0 while it iterates), written by
+  // the scene's first workgroup of that update
+  int active_v, active_p, active_cg, cg_stop_at;
   int iter_v, iter_p, iter_cg, status;
   unsigned counter, max_rho_bits;
   unsigned long long pairs, pairs_fluid;

@@ -111,6 +111,7 @@ __global__ void k_solver_begin(Dev d, int which) {
       st.err_p = 0.0;
     } else {
       st.active_cg = 0;
+      st.cg_stop_at = 0;
       st.iter_cg = 0;
       st.cg_rr = 0.0;
       st.cg_bb = 0.0;
@@ -450,6 +451,7 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_scene_finish(Dev d, int whic
     st->iter_cg = 0;
     const int act = (tot[0] > 0.0 && tot[1] > sc.cg_tol2 * tot[0]) ? 1 : 0;
     st->active_cg = act;
+    st->cg_stop_at = 0;
     if (act) atomicAdd(&d.n_active[2], 1);
   }
 }

@@ -248,7 +248,9 @@ __global__ void k_cg_matvec(Dev d, int start) {
   if (!quad_slot_scene(d, i, q, s, hw)) return;
   const SceneDev& sc = d.scenes[s];
   const SceneState* st = d.state + s;
-  if (!weiler_scene(sc) || !(start || st->active_cg)) return;
+  // the start product runs for every Weiler scene, an iteration's for the
+  // scenes that no update has stopped
+  if (!weiler_scene(sc) || !(start || (st->active_cg && st->cg_stop_at == 0))) return;
   double v[1] = {0.0};
   const bool valid = i < sc.p_begin + sc.n && kind_of(d.info[i]) == KIND_FLUID;
   float ax = 0.f, ay = 0.f, az = 0.f;
@@ -382,24 +384,25 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
 constexpr int UPD_THREADS = 64 * CG_UPD_WAVES;
 constexpr int UPD_WAVES = UPD_THREADS / 64;
 
-// The scene's first workgroup may clear active_cg in this same launch, so the
-// flag is read once per workgroup (one thread, through LDS): a workgroup whose
-// waves read it on both sides of that store would split, some waves leaving
-// and the rest reducing with wave sums that the others never wrote.  A
-// workgroup that still sees the flag set reaches the first workgroup's
-// decision from the same sums and leaves without writing.
+// Which scenes take part is read from state that no workgroup of this launch
+// changes in a way that matters to the others: active_cg is set by the start
+// reduction, and cg_stop_at by an earlier update -- or by this one's first
+// workgroup of the scene, as k, which reads the same as 0 here.  (Had that
+// workgroup cleared a flag that the others read, a workgroup whose waves read
+// it on both sides of the store would split at the barrier below, some waves
+// leaving and the rest reducing with wave sums never written.)  A workgroup of
+// a scene stopping now reaches the same decision from the same sums and
+// leaves without writing.
 __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
   __shared__ double red[3][UPD_WAVES];
-  __shared__ int active;
   const int2 e = d.cg_blocks[blockIdx.x];
   const int s = e.x;
   const SceneDev& sc = d.scenes[s];
   SceneState* st = d.state + s;
   const int t = threadIdx.x;
   if (blockIdx.x == 0 && t == 0) d.cg_list_n[(k + 1) & 1] = 0;  // for update k + 1
-  if (t == 0) active = st->active_cg;
-  __syncthreads();
-  if (!active) return;  // uniform over the workgroup
+  const int stop = st->cg_stop_at;
+  if (!st->active_cg || (stop != 0 && stop < k)) return;  // uniform over the workgroup
   const bool lead = e.y == sc.wave_begin && t == 0;
   double al, be;
   if (k == 1) {
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_cg_update(Dev d, int k) {
       if (lead) {
         st->iter_cg = it;
         st->cg_rr = rr;
-        st->active_cg = 0;
+        st->cg_stop_at = k;
         if (!conv) st->status |= ST_NOCONV;
         atomicSub(&d.n_active[2], 1);
       }

@@ -78,7 +78,8 @@ class ServingLoop:
     lanes = 1: one pool, ticked by one thread.  lanes = K: the scenes spread over
     K pools (nonnewton_amd.lanes.LanePool), each ticked by its own thread on its
     own stream with no barrier between them; an arriving request goes to the
-    least loaded lane.  The budget holds over all lanes together."""
+    lane with the least work assigned (particles x steps of its resident
+    requests).  The budget holds over all lanes together."""
 
     def __init__(self, budget_bytes: int = 200 << 30, block: int = 0, keep_results: bool = False, lanes: int = 1,
                  shapes=None):
@@ -145,7 +146,9 @@ class ServingLoop:
                     reject(nxt, f"needs {2 * fp} bytes to join, over the budget of {self.budget}")
                     continue
                 try:
-                    key = lp.admit(sc.params, sc.x, sc.kind, sc.v, n_elastic=sc.n_elastic)
+                    # to the lane with the least work assigned: particles x steps
+                    key = lp.admit(sc.params, sc.x, sc.kind, sc.v, cost=float(sc.n) * nxt.steps,
+                                   n_elastic=sc.n_elastic)
                 except _lib.NNError as e:  # out of grid, cell/neighbor overflow, bad kinds ...
                     reject(nxt, str(e))
                     continue

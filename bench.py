@@ -325,11 +325,13 @@ def run_lanes(scs, steps, warmup, lanes, block=0, shapes=None, region=None):
     lp.close()
     lat = [t for lane in out for t in lane[0]]
     # per tick k: every scene's statistics in workload order
-    names = out[0][1][0].dtype.names
+    names = next(o[1][0].dtype.names for o in out if o[1])
     stats = []
     for k in range(steps):
         row = [None] * len(scs)
         for l in range(lanes):
+            if not scene_of[l]:
+                continue
             a = out[l][1][k]
             for q, sk in enumerate(scene_of[l]):
                 row[sk] = {f: a[f][q].item() for f in names}
@@ -665,7 +667,7 @@ def main():
     n_all = sum(s.n for s in scs)
     log(f"[bench] workload {workload}: {len(scs)} scenes, {n_all} particles, {n_dyn} dynamic")
 
-    lanes = max(1, args.lanes) if len(scs) > 1 else 1
+    lanes = max(1, min(args.lanes, len(scs)))  # no lane without a scene
     b = run_batched(scs, args.steps, args.warmup, block=args.block, tune=not args.no_tune,
                     kernel=args.roofline_kernel,
                     region=TimedRegion() if (args.trace_region == "single" or lanes == 1) else _NoRegion())

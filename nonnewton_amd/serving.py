@@ -109,9 +109,9 @@ class ServingLoop:
         resident: Dict[Key, RequestRecord] = {}
         pending: List[RequestRecord] = list(order)
         state = {"stop": False, "freed": 0}
+        # peak_res: requests admitted and not yet finished (in HBM, joined or
+        # staged); peak_bytes: their footprints (the budget's accounting)
         acc = {"ticks": 0, "psteps": 0, "peak_res": 0, "peak_bytes": 0}
-        lane_res = [0] * K
-        lane_bytes = [0] * K
         tick_ms: List[float] = []
 
         def reject_locked(rec, why):  # with self._cond held
@@ -165,6 +165,8 @@ class ServingLoop:
                     nxt.slot = key
                     nxt.admitted = time.perf_counter() - t0
                     resident[key] = nxt
+                    acc["peak_res"] = max(acc["peak_res"], len(resident))
+                    acc["peak_bytes"] = max(acc["peak_bytes"], lp.bytes_in_use())
                     self._cond.notify_all()
 
         def guarded(fn, tag):
@@ -197,7 +199,6 @@ class ServingLoop:
                 t = time.perf_counter()
                 pool.tick()
                 dt = time.perf_counter() - t
-                r, q, b = pool.info()
                 # a scene admitted during the tick joins at the next one: count
                 # what the pool actually stepped
                 done = []
@@ -218,9 +219,6 @@ class ServingLoop:
                     tick_ms.append(1e3 * dt)
                     acc["ticks"] += 1
                     acc["psteps"] += psteps
-                    lane_res[l], lane_bytes[l] = r, b
-                    acc["peak_res"] = max(acc["peak_res"], sum(lane_res))
-                    acc["peak_bytes"] = max(acc["peak_bytes"], sum(lane_bytes))
                 for key in done:
                     rec = resident[key]
                     if self.keep_results:

@@ -1,6 +1,7 @@
 """Serving loop on the GPU: requests admitted while others run give the same bits
 as when run alone; a failing scene is evicted without perturbing the others; a
-tight memory budget queues requests until retirements free it."""
+tight memory budget queues requests until retirements free it.  Each case runs
+with one pool and with three lanes."""
 import numpy as np
 import pytest
 
@@ -28,9 +29,10 @@ def requests(steps=(3, 5, 2, 4), seed=3):
     return [Request(k, 0.02 * k, sc, st) for k, (sc, st) in enumerate(zip(batch, steps))]
 
 
-def test_serving_results_equal_alone():
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_results_equal_alone(lanes):
     reqs = requests()
-    loop = ServingLoop(8 << 30, keep_results=True)
+    loop = ServingLoop(8 << 30, keep_results=True, lanes=lanes)
     rep = loop.run(reqs)
     loop.close()
     s = rep.summary()
@@ -42,13 +44,14 @@ def test_serving_results_equal_alone():
     assert s["p99_step_ms"] >= s["p50_step_ms"] > 0
 
 
-def test_serving_evicts_failing_scene():
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_evicts_failing_scene(lanes):
     reqs = requests(steps=(3, 3, 3))
     bad = reqs[1].scene
     v = bad.v.copy()
     v[0] = np.nan  # forced from the host: the scene's state turns non-finite
     bad.v = v
-    loop = ServingLoop(8 << 30, keep_results=True)
+    loop = ServingLoop(8 << 30, keep_results=True, lanes=lanes)
     rep = loop.run(reqs)
     loop.close()
     assert rep.records[1].failed and rep.records[1].done_steps == 1
@@ -58,12 +61,13 @@ def test_serving_evicts_failing_scene():
         assert np.array_equal(rep.records[k].result["x"], ref["x"])
 
 
-def test_serving_budget_queues_requests():
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_budget_queues_requests(lanes):
     reqs = requests(steps=(2, 2, 2, 2), seed=5)
     pool = L.Pool(1 << 34)
     fp = max(pool.footprint(q.scene.params, q.scene.n, q.scene.n_elastic) for q in reqs)
     pool.close()
-    loop = ServingLoop(int(3.5 * fp), keep_results=True)
+    loop = ServingLoop(int(3.5 * fp), keep_results=True, lanes=lanes)
     rep = loop.run(reqs)
     loop.close()
     assert rep.peak_resident <= 2
@@ -72,7 +76,8 @@ def test_serving_budget_queues_requests():
         assert np.array_equal(rec.result["x"], alone(q.scene, q.steps)["x"])
 
 
-def test_serving_rejects_what_cannot_be_admitted():
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_rejects_what_cannot_be_admitted(lanes):
     """A request larger than the budget allows and one with a particle outside
     its grid are marked failed with a reason; the loop returns and the other
     requests are served as when alone."""
@@ -86,7 +91,7 @@ def test_serving_rejects_what_cannot_be_admitted():
     x = bad.x.copy()
     x[0] = (-1.0, 0.1, 0.1)  # below the domain: out of grid at admission
     bad.x = x
-    loop = ServingLoop(int(2.5 * fp), keep_results=True)
+    loop = ServingLoop(int(2.5 * fp), keep_results=True, lanes=lanes)
     rep = loop.run(reqs)
     loop.close()
     r = rep.records

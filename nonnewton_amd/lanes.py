@@ -40,14 +40,20 @@ class LanePool:
         # each lane may use the whole budget; admit() keeps the sum within it.
         # With several lanes every pool's stream gets a hardware queue of its
         # own: lanes that shared one would run one after the other.
-        self.pools = [_lib.Pool(self.budget, own_queue=lanes > 1) for _ in range(lanes)]
-        for p in self.pools:
-            if block:
-                p.set_block(block)
-            if shapes:
-                from . import autotune
+        self.pools: List[_lib.Pool] = []
+        try:
+            for _ in range(lanes):
+                self.pools.append(_lib.Pool(self.budget, own_queue=lanes > 1))
+            for p in self.pools:
+                if block:
+                    p.set_block(block)
+                if shapes:
+                    from . import autotune
 
-                autotune.apply(p, shapes)
+                    autotune.apply(p, shapes)
+        except BaseException:  # a pool that failed to be created or configured: release the others
+            self.close()
+            raise
         self.load = [0.0] * lanes
         self.cost: Dict[Key, float] = {}
         self.bytes: Dict[Key, int] = {}

@@ -1,5 +1,5 @@
 """The FP32 HIP step against the FP64 oracle (tests/fp64_error.py computes the
-table; DESIGN.md holds it): per-field single-step error for the dam break,
+error table): per-field single-step error for the dam break,
 the non-Newtonian dam break and the elastic drop, from the initial state and
 late in a run, and the 100-step drift of the average density error."""
 import numpy as np

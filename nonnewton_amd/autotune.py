@@ -29,8 +29,8 @@ from typing import Dict, Iterable, Optional
 from . import _lib
 
 CANDIDATES = (64, 128, 256, 512, 1024)
-# kernels with a fixed shape: the scan, and the CG update (scene-aligned
-# 1024-thread workgroups)
+# kernels with a fixed shape: the scan, and the CG update (workgroups of
+# whole waves of one scene)
 FIXED = {"scan", "cg_update"}
 
 
@@ -68,9 +68,12 @@ def _store_cache(c: dict) -> None:
 
 
 # gather depths each kernel implements (see nn_pool_set_kernel_depth)
-DEPTHS = {"cg_matvec": (1, 2, 4, 8)}
-DEPTHS.update({k: (4, 8, 16) for k in ("density_alpha", "div_update", "div_source", "press_source",
-                                       "press_update", "nonpressure", "strain")})
+# The CG matvec is not tuned: it runs depths 1, 2, 4 and 8, and with its list
+# words requested a batch ahead its default of 2 was the fastest in one pool
+# and in lanes, where a depth picked from one pool's per-launch times (1) made
+# whole ticks slower.
+DEPTHS = {k: (4, 8, 16) for k in ("density_alpha", "div_update", "div_source", "press_source",
+                                  "press_update", "nonpressure", "strain")}
 
 
 def apply(pool, shapes: Dict[str, Dict[str, int]]) -> None:

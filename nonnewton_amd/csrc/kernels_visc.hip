@@ -239,8 +239,10 @@ __global__ void k_cg_guess(Dev d) {
 // otherwise: an iteration, active scenes; read by the next update).  D list
 // positions of a lane per batch: their list words and pair scalars, then their
 // neighbors' records (position and u, 24 bytes), are loaded before any of them is used (a
-// shorter chain of dependent loads); the terms are added in list order
-// whatever D is.  A pair with s_ij = 0 -- every neighbor that is not fluid --
+// shorter chain of dependent loads), and the next batch's list words and pair
+// scalars are requested right after this batch's records, so that they arrive
+// while this batch is summed; the terms are added in list order whatever D is.
+// A pair with s_ij = 0 -- every neighbor that is not fluid --
 // adds nothing and is skipped, so the neighbor's u is never read there.
 template <int D>
 __global__ void k_cg_matvec(Dev d, int start) {
@@ -261,21 +263,37 @@ __global__ void k_cg_matvec(Dev d, int start) {
     ui = own.u;
     const int cnt = d.nnbr[i];
     const long P = d.P;
+    int jn[D];
+    float sn[D];
+#pragma unroll
+    for (int e = 0; e < D; e++) {
+      const bool in = q + 4 * e < cnt;
+      jn[e] = in ? d.nbr[nb_word(P, i, q + 4 * e)] : i;
+      sn[e] = in ? d.spair[nb_word(P, i, q + 4 * e)] : 0.0f;
+    }
     for (int k = q; k < cnt; k += 4 * D) {
       int j[D];
       float sp[D];
       float3 xj[D], uj[D];
 #pragma unroll
       for (int e = 0; e < D; e++) {
-        const bool in = k + 4 * e < cnt;
-        j[e] = in ? d.nbr[nb_word(P, i, k + 4 * e)] : i;
-        sp[e] = in ? d.spair[nb_word(P, i, k + 4 * e)] : 0.0f;
+        j[e] = jn[e];
+        sp[e] = sn[e];
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
         const XU nb = d.xu[j[e]];
         xj[e] = nb.p;
         uj[e] = nb.u;
+      }
+      // the next batch's list words and pair scalars, in flight while this
+      // batch's records arrive
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        const int kk = k + 4 * D + 4 * e;
+        const bool in = kk < cnt;
+        jn[e] = in ? d.nbr[nb_word(P, i, kk)] : i;
+        sn[e] = in ? d.spair[nb_word(P, i, kk)] : 0.0f;
       }
 #pragma unroll
       for (int e = 0; e < D; e++) {
@@ -327,21 +345,35 @@ __global__ __launch_bounds__(MVL_THREADS) void k_cg_matvec_list(Dev d, int k) {
       ui = own.u;
       const int cnt = d.nnbr[i];
       const long P = d.P;
+      int jn[D];
+      float sn[D];
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        const bool in = q + 4 * e < cnt;
+        jn[e] = in ? d.nbr[nb_word(P, i, q + 4 * e)] : i;
+        sn[e] = in ? d.spair[nb_word(P, i, q + 4 * e)] : 0.0f;
+      }
       for (int kk = q; kk < cnt; kk += 4 * D) {
         int j[D];
         float sp[D];
         float3 xj[D], uj[D];
 #pragma unroll
         for (int e = 0; e < D; e++) {
-          const bool in = kk + 4 * e < cnt;
-          j[e] = in ? d.nbr[nb_word(P, i, kk + 4 * e)] : i;
-          sp[e] = in ? d.spair[nb_word(P, i, kk + 4 * e)] : 0.0f;
+          j[e] = jn[e];
+          sp[e] = sn[e];
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {
           const XU nb = d.xu[j[e]];
           xj[e] = nb.p;
           uj[e] = nb.u;
+        }
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          const int k2 = kk + 4 * D + 4 * e;
+          const bool in = k2 < cnt;
+          jn[e] = in ? d.nbr[nb_word(P, i, k2)] : i;
+          sn[e] = in ? d.spair[nb_word(P, i, k2)] : 0.0f;
         }
 #pragma unroll
         for (int e = 0; e < D; e++) {

@@ -112,6 +112,27 @@ def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
         assert len(json.load(fh)) == 1
 
 
+@pytest.mark.parametrize("depth", [1, 4, 8])
+def test_matvec_depths_give_the_same_bits(depth):
+    """The CG matvec at every gather depth (list entries per lane and batch,
+    the next batch's list words requested while a batch is summed) steps a
+    mixed pool to the bits of the default depth."""
+    batch = scenes.mixed_batch(count=3, lo=2000, hi=5000)
+    out = []
+    for dep in (0, depth):
+        pool = L.Pool(8 << 30)
+        slots = [pool.admit(s.params, s.x, s.kind, s.v) for s in batch]
+        pool.set_kernel_depth("cg_matvec", dep)
+        for _ in range(6):
+            pool.tick()
+        out.append([pool.read(q, f) for q in slots for f in (L.FIELD_X, L.FIELD_V)])
+        its = [pool.stats(q)["iterations_cg"] for q in slots]
+        pool.close()
+    assert max(its) > 1
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_pool_newtonian_scene_matches_oracle():
     rng = np.random.Generator(np.random.PCG64(21))
     sc = scenes.mixed_scene(rng, 2500, 0, 1.5, 1.0, False, "newt", dt=5e-4)

@@ -223,19 +223,20 @@ def test_mixed_batch_64_scenes():
 
 
 @pytest.mark.timeout(600)
-def test_serving_stream_real_sizes():
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_stream_real_sizes(lanes):
     """The serving stream's first 24 requests (seed 4: 1k-500k particles, the
     largest above 200k), each cut to 3 steps and arriving 100 times faster than
     at 20 requests/s (so that requests overlap), through the serving loop under
-    the 200 GB budget; the largest, the smallest and one elastic request give
-    the same bits as when stepped alone."""
+    the 200 GB budget, in one pool and in three lanes; the largest, the
+    smallest and one elastic request give the same bits as when stepped alone."""
     reqs = scenes.serving_requests(count=24)
     reqs = [Request(q.rid, q.arrival, q.scene, 3) for q in reqs]
     sizes = [q.scene.n for q in reqs]
     assert max(sizes) >= 200_000
     from nonnewton_amd.serving import ServingLoop
 
-    loop = ServingLoop(200 << 30, keep_results=True)
+    loop = ServingLoop(200 << 30, keep_results=True, lanes=lanes)
     rep = loop.run(reqs, time_scale=0.01)
     loop.close()
     assert all(r.finished is not None and not r.failed for r in rep.records)

@@ -116,8 +116,9 @@ def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
 def test_matvec_depths_give_the_same_bits(depth):
     """The CG matvec at every gather depth (list entries per lane and batch,
     the next batch's list words requested while a batch is summed) steps a
-    mixed pool to the bits of the default depth."""
-    batch = scenes.mixed_batch(count=3, lo=2000, hi=5000)
+    mixed pool to the bits of the default depth (six scenes that converge at
+    different counts, so the late iterations may run the listed matvec)."""
+    batch = scenes.mixed_batch(count=6, lo=2000, hi=5000)
     out = []
     for dep in (0, depth):
         pool = L.Pool(8 << 30)

@@ -66,6 +66,29 @@ def test_models_3_to_6_match_oracle(method):
         assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-12), name
 
 
+def test_models_3_to_6_in_lanes_equal_alone():
+    """Casson, Carreau, Bingham and Herschel-Bulkley scenes spread over two
+    lanes: each scene's state after 4 ticks equals its run alone, bit for bit
+    (the same device arithmetic in both)."""
+    from nonnewton_amd.lanes import LanePool
+
+    todo = [nonnewton_scene(m, **MODEL_PARAMS[m]) for m in sorted(MODEL_PARAMS)]
+    lp = LanePool(8 << 30, lanes=2)
+    keys = [lp.admit(s.params, s.x, s.kind, s.v, n_elastic=s.n_elastic) for s in todo]
+    assert {k[0] for k in keys} == {0, 1}
+    lp.run(4, [[] for _ in range(2)])
+    for sc, key in zip(todo, keys):
+        ctx = L.Context(sc.params, sc.n)
+        ctx.upload(sc.x, sc.kind, sc.v)
+        ctx.init()
+        its = [ctx.step()["iterations_cg"] for _ in range(4)]
+        assert lp.stats(key)["iterations_cg"] == its[-1] > 0
+        for f in (L.FIELD_X, L.FIELD_V, L.FIELD_VISCOSITY):
+            assert np.array_equal(lp.read(key, f), ctx.download(f)), (sc.name, f)
+        ctx.close()
+    lp.close()
+
+
 def fluid_against_elastic(seed=21, **kw):
     """A jittered 8^3 fluid block against a 4 x 8 x 8 elastic block, one
     spacing apart, in a walled box: fluid-elastic pairs from the first step."""

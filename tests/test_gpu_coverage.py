@@ -206,7 +206,9 @@ def test_mixed_batch_64_scenes():
     error of the pressure solve within 0.1% of rho0; after 2 ticks two
     Newtonian scenes with elastic blocks equal the FP32 oracle bit for bit and
     a power-law scene within 1e-5; after 6 ticks the scenes that set the CG tail
-    (13, 33, 34: power law with n < 0.7) equal themselves stepped alone."""
+    (13, 33, 34: power law with n < 0.7) equal themselves stepped alone, and
+    every scene equals itself stepped in three lanes (admitted largest first,
+    as the bench does)."""
     batch = scenes.mixed_batch()
     assert len(batch) == 64
     pool = L.Pool(200 << 30)
@@ -242,6 +244,16 @@ def test_mixed_batch_64_scenes():
         for f, arr in ref.items():
             assert np.array_equal(pool.read(slots[k], f), arr), (k, f)
         assert pool.stats(slots[k])["iterations_cg"] == st["iterations_cg"]
+    from nonnewton_amd.lanes import LanePool
+
+    lp = LanePool(200 << 30, lanes=3)
+    keys = {k: lp.admit(batch[k].params, batch[k].x, batch[k].kind, batch[k].v, n_elastic=batch[k].n_elastic)
+            for k in sorted(range(64), key=lambda k: (-batch[k].n, k))}
+    lp.run(6, [[] for _ in range(3)])
+    for k in range(64):
+        for f in (L.FIELD_X, L.FIELD_V):
+            assert np.array_equal(lp.read(keys[k], f), pool.read(slots[k], f)), (k, f)
+    lp.close()
     pool.close()
 
 
